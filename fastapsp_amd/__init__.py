@@ -1,0 +1,287 @@
+"""fastapsp_amd -- MI355X (gfx950) subgraph-solve path of FastAPSP.
+
+Python access to libfap_amd.so through ctypes: the C ABI of include/fap_dev.h
+(device-pointer kernels), include/fap_graph.h (the fap::fapGraph drop-in) and
+include/fap_ops.h (the reference's host float operators).  numpy arrays are
+the host format; device memory is managed by the library itself
+(``Context.alloc``), so no other GPU runtime is involved.
+
+The library must have been built (``python fastapsp_amd/build.py`` or
+``__graft_entry__.build()``); importing the compute API without it raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+INF = 0x3FFFFFFF
+REF_INF = 1e8
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libfap_amd.so")
+
+_i32, _i64, _u64, _int = C.c_int32, C.c_int64, C.c_uint64, C.c_int
+_vp, _sz = C.c_void_p, C.c_size_t
+_pi32 = C.POINTER(C.c_int32)
+_pf32 = C.POINTER(C.c_float)
+_pu64 = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "fap_last_error": (C.c_char_p, []),
+    "fap_version": (C.c_char_p, []),
+    "fap_device_count": (_int, [C.POINTER(_int)]),
+    "fap_ctx_create": (_int, [_int, C.POINTER(_vp)]),
+    "fap_ctx_destroy": (_int, [_vp]),
+    "fap_ctx_stream": (_int, [_vp, C.POINTER(_vp)]),
+    "fap_ctx_sync": (_int, [_vp]),
+    "fap_dev_alloc": (_int, [_vp, _sz, C.POINTER(_vp)]),
+    "fap_dev_free": (_int, [_vp, _vp]),
+    "fap_memcpy_h2d": (_int, [_vp, _vp, _vp, _sz]),
+    "fap_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
+    "fap_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
+    "fap_fill_i32": (_int, [_vp, _vp, _i64, _i32, _vp]),
+    "fap_checksum_i32": (_int, [_vp, _vp, _i64, _pu64, _pu64, _vp]),
+    "fap_stream_create": (_int, [_vp, C.POINTER(_vp)]),
+    "fap_stream_destroy": (_int, [_vp, _vp]),
+    "fap_stream_sync": (_int, [_vp]),
+    "fap_event_create": (_int, [C.POINTER(_vp)]),
+    "fap_event_destroy": (_int, [_vp]),
+    "fap_event_record": (_int, [_vp, _vp]),
+    "fap_event_elapsed": (_int, [_vp, _vp, C.POINTER(C.c_float)]),
+    "fap_graph_upload": (_int, [_vp, _i32, _i64, _pi32, _pi32, _pi32, C.POINTER(_vp)]),
+    "fap_graph_set_partition": (_int, [_vp, _vp, _pi32, _pi32, _pi32]),
+    "fap_graph_free": (_int, [_vp, _vp]),
+    "fap_graph_info": (_int, [_vp, _pi32, C.POINTER(_i64)]),
+    "fap_sssp_multi": (_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp]),
+    "fap_fw_blocked": (_int, [_vp, _vp, _i32, _i64, _vp]),
+    "fap_minplus": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _int, _vp]),
+    "fap_gather_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
+    "fap_scatter_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
+    "fap_subgraph_solve": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp]),
+    # fap_ops.h
+    "fap_handle_boundry_path": (_int, [_pf32, _pi32, _int, _int, _int, _pi32, _pi32, _pi32,
+                                       _pf32, _pi32, _int]),
+    "fap_floyd_path": (_int, [_int, _pf32, _pi32]),
+    "fap_min_plus_path_advanced": (_int, [_pf32, _pf32, _pi32, _pf32, _pi32, _int, _int, _int]),
+    # fap_graph.h
+    "fapg_create_from_file": (_int, [C.c_char_p, _int, _int, _i32, C.POINTER(_vp)]),
+    "fapg_create_from_csr": (_int, [_i32, _i64, _pi32, _pi32, _pf32, _i32, C.POINTER(_vp)]),
+    "fapg_destroy": (_int, [_vp]),
+    "fapg_set_partition": (_int, [_vp, _pi32]),
+    "fapg_precondition": (_int, [_vp]),
+    "fapg_is_split": (_int, [_vp, C.POINTER(_int)]),
+    "fapg_solve": (_int, [_vp, _int]),
+    "fapg_solve_subgraph": (_int, [_vp, _i32, _int]),
+    "fapg_num_vertices": (_int, [_vp, _pi32]),
+    "fapg_num_edges": (_int, [_vp, C.POINTER(_i64)]),
+    "fapg_num_parts": (_int, [_vp, _pi32]),
+    "fapg_is_directed": (_int, [_vp, C.POINTER(_int)]),
+    "fapg_used_metis": (_int, [_vp, C.POINTER(_int)]),
+    "fapg_get_csr": (_int, [_vp, _pi32, _pi32, _pi32]),
+    "fapg_get_graph_id": (_int, [_vp, _pi32]),
+    "fapg_get_boundary_flags": (_int, [_vp, _pi32]),
+    "fapg_get_mapping": (_int, [_vp, _pi32, _pi32]),
+    "fapg_get_part_info": (_int, [_vp, _pi32, _pi32, _pi32, _pi32]),
+    "fapg_get_current_subgraph_id": (_int, [_vp, _pi32]),
+    "fapg_get_subgraph_index": (_int, [_vp, _i32, _pi32]),
+    "fapg_get_subgraph_distance_i32": (_int, [_vp, _i32, _pi32]),
+    "fapg_get_subgraph_distance_f32": (_int, [_vp, _i32, _pf32]),
+    "fapg_get_all_distance_i32": (_int, [_vp, _pi32]),
+    "fapg_get_all_distance_f32": (_int, [_vp, _pf32]),
+    "fapg_get_distance_from_one_point": (_int, [_vp, _i32, _pf32]),
+    "fapg_distance_p2p": (_int, [_vp, _i32, _i32, _pi32]),
+    "fapg_subgraph_checksum": (_int, [_vp, _i32, _pu64, _pu64]),
+    "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
+}
+
+_lib = None
+
+
+class FapError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load(path: str | None = None):
+    """Load libfap_amd.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FapError(-4, f"{p} is missing: build it with `python fastapsp_amd/build.py`")
+    lib = C.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().fap_last_error().decode()
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        raise FapError(rc, f"{what}: {last_error()}" if what else last_error())
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(_pi32)
+
+
+def _pf(a: np.ndarray):
+    return a.ctypes.data_as(_pf32)
+
+
+class DeviceArray:
+    """int32 device buffer owned by a Context (shape is host bookkeeping)."""
+
+    def __init__(self, ctx: "Context", shape, ptr: int):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.ptr = ptr
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape)) if self.shape else 1
+
+    @property
+    def nbytes(self) -> int:
+        return 4 * self.size
+
+    def offset(self, elems: int) -> int:
+        return self.ptr + 4 * int(elems)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.int32)
+        if out.size:
+            check(load().fap_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def set(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.int32)
+        assert h.size == self.size, (h.shape, self.shape)
+        if h.size:
+            check(load().fap_memcpy_h2d(self.ctx.handle, self.ptr, h.ctypes.data, h.nbytes), "h2d")
+
+    def free(self):
+        if self.ptr:
+            load().fap_dev_free(self.ctx.handle, self.ptr)
+            self.ptr = 0
+
+
+class Context:
+    """A device context (fap_ctx): stream, workspaces, device memory."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = _vp()
+        check(lib.fap_ctx_create(device, C.byref(h)), "fap_ctx_create")
+        self.handle = h.value
+        s = _vp()
+        check(lib.fap_ctx_stream(self.handle, C.byref(s)), "fap_ctx_stream")
+        self.stream = s.value
+
+    def close(self):
+        if self.handle:
+            load().fap_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(load().fap_ctx_sync(self.handle), "sync")
+
+    def alloc(self, shape) -> DeviceArray:
+        shape = (shape,) if isinstance(shape, (int, np.integer)) else tuple(shape)
+        n = int(np.prod(shape)) if shape else 1
+        p = _vp()
+        check(load().fap_dev_alloc(self.handle, max(4, 4 * n), C.byref(p)), "alloc")
+        return DeviceArray(self, shape, p.value)
+
+    def to_device(self, host: np.ndarray) -> DeviceArray:
+        h = np.ascontiguousarray(host, dtype=np.int32)
+        d = self.alloc(h.shape)
+        d.set(h)
+        return d
+
+    def fill(self, d: DeviceArray, value: int, n: int | None = None, ptr: int | None = None):
+        check(load().fap_fill_i32(self.handle, ptr or d.ptr, d.size if n is None else n,
+                                  int(value), None), "fill")
+
+    def checksum(self, ptr: int, n: int):
+        s, w = C.c_uint64(), C.c_uint64()
+        check(load().fap_checksum_i32(self.handle, ptr, n, C.byref(s), C.byref(w), None), "checksum")
+        return s.value, w.value
+
+    # ---- kernels (device pointers; see fap_dev.h) ----
+    def minplus(self, A: int, lda: int, B: int, ldb: int, C_: int, ldc: int, M: int, N: int,
+                K: int, accumulate: bool = False, stream=None):
+        check(load().fap_minplus(self.handle, A, lda, B, ldb, C_, ldc, M, N, K,
+                                 1 if accumulate else 0, stream), "fap_minplus")
+
+    def fw(self, ptr: int, n: int, ld: int, stream=None):
+        check(load().fap_fw_blocked(self.handle, ptr, n, ld, stream), "fap_fw_blocked")
+
+    def upload_graph(self, row_offset, col, w) -> "DeviceGraph":
+        return DeviceGraph(self, row_offset, col, w)
+
+
+class DeviceGraph:
+    """A CSR resident in HBM (fap_graph_dev)."""
+
+    def __init__(self, ctx: Context, row_offset, col, w):
+        self.ctx = ctx
+        ro = np.ascontiguousarray(row_offset, dtype=np.int32)
+        c = np.ascontiguousarray(col, dtype=np.int32)
+        ww = np.ascontiguousarray(w, dtype=np.int32)
+        self.V = ro.size - 1
+        self.E = c.size
+        h = _vp()
+        check(load().fap_graph_upload(ctx.handle, self.V, self.E, _p32(ro), _p32(c), _p32(ww),
+                                      C.byref(h)), "fap_graph_upload")
+        self.handle = h.value
+
+    def set_partition(self, graph_id, st2ed, ed2st):
+        g = np.ascontiguousarray(graph_id, dtype=np.int32)
+        s = np.ascontiguousarray(st2ed, dtype=np.int32)
+        e = np.ascontiguousarray(ed2st, dtype=np.int32)
+        check(load().fap_graph_set_partition(self.ctx.handle, self.handle, _p32(g), _p32(s),
+                                             _p32(e)), "fap_graph_set_partition")
+
+    def sssp(self, d_sources: int, n_src: int, d_out: int, ld_out: int, stream=None):
+        check(load().fap_sssp_multi(self.ctx.handle, self.handle, d_sources, n_src, d_out,
+                                    ld_out, stream), "fap_sssp_multi")
+
+    def gather(self, start, n_sub, n_bdy, rowblock, ld_rb, sub, ld_sub, stream=None):
+        check(load().fap_gather_submat(self.ctx.handle, self.handle, start, n_sub, n_bdy,
+                                       rowblock, ld_rb, sub, ld_sub, stream), "gather")
+
+    def scatter(self, start, rows, n_sub, sub, ld_sub, rowblock, ld_rb, stream=None):
+        check(load().fap_scatter_submat(self.ctx.handle, self.handle, start, rows, n_sub, sub,
+                                        ld_sub, rowblock, ld_rb, stream), "scatter")
+
+    def subgraph_solve(self, start, n_sub, n_bdy, rowblock, ld_rb, stream=None):
+        check(load().fap_subgraph_solve(self.ctx.handle, self.handle, start, n_sub, n_bdy,
+                                        rowblock, ld_rb, stream), "fap_subgraph_solve")
+
+    def close(self):
+        if self.handle:
+            load().fap_graph_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+from .graph import Graph  # noqa: E402  (the fap::fapGraph drop-in)
+from . import ops  # noqa: E402,F401

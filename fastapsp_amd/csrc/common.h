@@ -1,0 +1,90 @@
+// common.h -- internal helpers shared by the HIP and host translation units.
+#ifndef FAP_COMMON_H_
+#define FAP_COMMON_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fap_dev.h"
+
+namespace fapi {
+
+constexpr int32_t kInf = FAP_INF;
+
+// Thread-local last error; every C entry point returns a status and leaves
+// the message here.
+void set_error(const std::string &msg);
+void clear_error();
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define FAP_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) return fapi::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define FAP_CHECK_LAUNCH()                                             \
+  do {                                                                 \
+    hipError_t e_ = hipGetLastError();                                 \
+    if (e_ != hipSuccess) return fapi::hip_fail(e_, "kernel launch", __FILE__, __LINE__); \
+  } while (0)
+
+#define FAP_TRY(call)                                                  \
+  do {                                                                 \
+    int rc_ = (call);                                                  \
+    if (rc_ != FAP_OK) return rc_;                                     \
+  } while (0)
+
+// Device buffer grown on demand (never shrinks); owned by the context.
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace fapi
+
+struct fap_graph_dev {
+  int32_t V = 0;
+  int64_t E = 0;
+  int32_t *row_offset = nullptr;  // V+1
+  int2 *edges = nullptr;          // E x {col, w}
+  int32_t *graph_id = nullptr;    // V (after set_partition)
+  int32_t *st2ed = nullptr;       // V
+  int32_t *ed2st = nullptr;       // V
+  int32_t max_degree = 0;
+};
+
+struct fap_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  size_t lds_per_block = 0;
+  std::mutex mu;  // serialises workspace growth
+  // named workspaces (device)
+  fapi::Workspace ws_sub;     // dense part matrix
+  fapi::Workspace ws_queue;   // SSSP frontier queues
+  fapi::Workspace ws_stamp;   // SSSP enqueue stamps
+  fapi::Workspace ws_small;   // counters etc.
+  fapi::Workspace ws_misc;
+  uint32_t *host_pinned = nullptr;  // small pinned host buffer for counters
+  hipEvent_t ev_poll = nullptr;
+};
+
+namespace fapi {
+// Ensure ws holds >= bytes (device memory of ctx's device). Synchronizes the
+// context stream before reallocating.
+int ensure_ws(fap_ctx *ctx, Workspace &ws, size_t bytes);
+inline hipStream_t pick_stream(fap_ctx *ctx, void *stream) {
+  return stream ? (hipStream_t)stream : ctx->stream;
+}
+// Device-side helpers implemented in util.hip.
+int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
+int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);
+}  // namespace fapi
+
+#endif  // FAP_COMMON_H_

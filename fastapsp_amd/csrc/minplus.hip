@@ -1,0 +1,81 @@
+// minplus.hip -- min-plus product C = min(base, A (x) B) on the VALU.
+//
+// Replaces the reference's stage-3 operator (min_plus_path_advanced,
+// fap/kernel/minplus.h:78-89; CPU semantics :51-65: every output starts at
+// INF and takes the strict minimum over k).  One 128 x 128 tile per
+// workgroup, see minplus_tile.h for the register/LDS layout.
+#include "common.h"
+#include "minplus_tile.h"
+
+namespace fapk {
+
+// Tile order: workgroups of one XCD take a contiguous tile range; inside it,
+// groups of 8 row tiles walk the column panels together so that a B panel
+// (K x 128) is reused by the 8 tiles that need it while it is in L2.
+__device__ __forceinline__ void gemm_tile_coords(int bid, int tiles_m, int tiles_n, int *tm,
+                                                 int *tn) {
+  const int total = tiles_m * tiles_n;
+  const int t = xcd_remap(bid, total);
+  const int G = 8;
+  const int group = t / (G * tiles_n);
+  const int first_m = group * G;
+  const int gsize = imin(G, tiles_m - first_m);
+  const int within = t - group * G * tiles_n;
+  *tm = first_m + within % gsize;
+  *tn = within / gsize;
+}
+
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void minplus_gemm_kernel(
+    const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C, int64_t ldc,
+    int M, int N, int K, int accumulate, int flags, int tiles_m, int tiles_n) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[MP_LDS_WORDS];
+  int tm, tn;
+  gemm_tile_coords(blockIdx.x, tiles_m, tiles_n, &tm, &tn);
+  const int m0 = tm * MP_BM, n0 = tn * MP_BN;
+  minplus_tile(A + (int64_t)m0 * lda, lda, B + n0, ldb, C + (int64_t)m0 * ldc + n0, ldc,
+               imin(MP_BM, M - m0), imin(MP_BN, N - n0), K, accumulate != 0, flags, lds);
+}
+
+}  // namespace fapk
+
+namespace fapi {
+
+int vec_flags(const void *A, int64_t lda, const void *B, int64_t ldb, const void *C,
+              int64_t ldc) {
+  int f = 0;
+  if (((uintptr_t)A & 15) == 0 && (lda & 3) == 0) f |= fapk::MP_VEC_A;
+  if (((uintptr_t)B & 15) == 0 && (ldb & 3) == 0) f |= fapk::MP_VEC_B;
+  if (((uintptr_t)C & 15) == 0 && (ldc & 3) == 0) f |= fapk::MP_VEC_C;
+  return f;
+}
+
+int launch_minplus(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C,
+                   int64_t ldc, int32_t M, int32_t N, int32_t K, int accumulate,
+                   hipStream_t s) {
+  if (M <= 0 || N <= 0) return FAP_OK;
+  const int64_t tiles_m = (M + fapk::MP_BM - 1) / fapk::MP_BM;
+  const int64_t tiles_n = (N + fapk::MP_BN - 1) / fapk::MP_BN;
+  if (tiles_m * tiles_n > INT32_MAX) return fail(FAP_E_ARG, "min-plus: too many tiles");
+  const int flags = vec_flags(A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL(fapk::minplus_gemm_kernel, dim3((unsigned)(tiles_m * tiles_n)),
+                     dim3(fapk::MP_THREADS), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N,
+                     (int)K, accumulate, flags, (int)tiles_m, (int)tiles_n);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+}  // namespace fapi
+
+using namespace fapi;
+
+extern "C" int fap_minplus(fap_ctx *ctx, const int32_t *A, int64_t lda, const int32_t *B,
+                           int64_t ldb, int32_t *C, int64_t ldc, int32_t M, int32_t N,
+                           int32_t K, int accumulate, void *stream) {
+  if (!ctx) return fail(FAP_E_ARG, "ctx is null");
+  if (M < 0 || N < 0 || K < 0) return fail(FAP_E_ARG, "min-plus: negative size");
+  if (M == 0 || N == 0) return FAP_OK;
+  if (!C || (K > 0 && (!A || !B))) return fail(FAP_E_ARG, "min-plus: null operand");
+  if (ldc < N || (K > 0 && (lda < K || ldb < N)))
+    return fail(FAP_E_ARG, "min-plus: leading dimension smaller than the row length");
+  return launch_minplus(A, lda, B, ldb, C, ldc, M, N, K, accumulate, pick_stream(ctx, stream));
+}

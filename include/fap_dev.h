@@ -1,0 +1,147 @@
+/*
+ * fap_dev.h -- C ABI of the MI355X (gfx950) subgraph-solve path.
+ *
+ * Plain C: opaque handles, int32 data, explicit leading dimensions, device
+ * pointers, a `void *stream` (hipStream_t; NULL = the context's own stream).
+ * Every function returns FAP_OK (0) or a negative FAP_E* code; the message of
+ * the last failure on the calling thread is fap_last_error().  Calls are
+ * asynchronous on the stream unless stated otherwise.
+ *
+ * Numbers: distances and weights are int32 >= 0, INF = FAP_INF = INT32_MAX/2,
+ * so INF + INF does not overflow.  The reference computes in float with
+ * INF = 1e8f (fap/utils/parameter.h:26); results agree exactly while every
+ * finite distance is < 2^24.
+ *
+ * Reference interfaces replaced (paths relative to the FastAPSP repository):
+ *   fap_sssp_multi      <- handle_boundry_path      fap/kernel/batch_sssp.h:254-277
+ *                          handle_boundry_AMD_GPU   fap/kernel/hip/hip_kernel.h:17-20
+ *   fap_fw_blocked      <- floyd_path               fap/kernel/floyd.h:150-158
+ *                          floyd_AMD_path           fap/kernel/hip/hip_kernel.h:24
+ *   fap_minplus         <- min_plus_path_advanced   fap/kernel/minplus.h:78-89
+ *                          minplus_AMD_path         fap/kernel/hip/hip_kernel.h:40-41
+ *   fap_gather_submat   <- MysubMatBuild_path       fap/utils/util_Myalgorithm.h:30-73
+ *   (A operand view)    <- MyMat1Build              fap/utils/util_Myalgorithm.h:88-95
+ *   fap_scatter_submat  <- MysubMatDecode_path      fap/utils/util_Myalgorithm.h:75-86
+ *   fap_subgraph_solve  <- fapGraph::run            fap/fap.cpp:92-181
+ *   fap_batch_*         <- the MPI multi-task driver fap/fap_more_task_mpi.cpp:34-91,
+ *                          collapsed into one device work queue on one GPU
+ */
+#ifndef FAP_DEV_H_
+#define FAP_DEV_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FAP_INF 0x3FFFFFFF
+
+enum {
+  FAP_OK = 0,
+  FAP_E_ARG = -1,     /* invalid argument (sizes, null pointers, ranges) */
+  FAP_E_HIP = -2,     /* HIP runtime error (message has the HIP string) */
+  FAP_E_NOMEM = -3,   /* device or host allocation failed / over budget */
+  FAP_E_STATE = -4,   /* call out of order (e.g. solve before preCondition) */
+  FAP_E_IO = -5,      /* file missing or malformed */
+  FAP_E_DEVICE = -6,  /* a device-side bounded wait gave up (reported, not hung) */
+};
+
+typedef struct fap_ctx fap_ctx;
+typedef struct fap_graph_dev fap_graph_dev;
+
+/* Last error message of the calling thread ("" if none). */
+const char *fap_last_error(void);
+/* Library version string. */
+const char *fap_version(void);
+
+/* ---- context, memory and stream helpers (synchronous unless noted) ---- */
+int fap_device_count(int *count);
+int fap_ctx_create(int device, fap_ctx **out);
+int fap_ctx_destroy(fap_ctx *ctx);
+int fap_ctx_stream(fap_ctx *ctx, void **stream);
+int fap_ctx_sync(fap_ctx *ctx);
+int fap_dev_alloc(fap_ctx *ctx, size_t bytes, void **dptr);
+int fap_dev_free(fap_ctx *ctx, void *dptr);
+int fap_memcpy_h2d(fap_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fap_memcpy_d2h(fap_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fap_memcpy_d2d(fap_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Fill n int32 with value (a kernel, not hipMemset; asynchronous). */
+int fap_fill_i32(fap_ctx *ctx, int32_t *d, int64_t n, int32_t value, void *stream);
+/* Wrapping sum of n uint32 and of (index+1)*value (a position-keyed checksum);
+ * synchronous. */
+int fap_checksum_i32(fap_ctx *ctx, const int32_t *d, int64_t n, uint64_t *sum,
+                     uint64_t *weighted, void *stream);
+int fap_stream_create(fap_ctx *ctx, void **stream);
+int fap_stream_destroy(fap_ctx *ctx, void *stream);
+int fap_stream_sync(void *stream);
+/* Events for timing: record on a stream, elapsed milliseconds between two. */
+int fap_event_create(void **ev);
+int fap_event_destroy(void *ev);
+int fap_event_record(void *ev, void *stream);
+int fap_event_elapsed(void *start, void *stop, float *ms);
+
+/* ---- resident graph ---- */
+/* Upload a CSR (host arrays: row_offset[V+1], col[E], w[E], int32, w >= 0).
+ * Synchronous.  The graph stays in HBM until fap_graph_free. */
+int fap_graph_upload(fap_ctx *ctx, int32_t V, int64_t E,
+                     const int32_t *row_offset, const int32_t *col,
+                     const int32_t *w, fap_graph_dev **out);
+/* Attach the partition: graph_id (1-based parts), st2ed, ed2st (host, V
+ * each), as produced by preCondition.  Synchronous. */
+int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g,
+                            const int32_t *graph_id, const int32_t *st2ed,
+                            const int32_t *ed2st);
+int fap_graph_free(fap_ctx *ctx, fap_graph_dev *g);
+int fap_graph_info(const fap_graph_dev *g, int32_t *V, int64_t *E);
+
+/* ---- the three stages ---- */
+/* Multi-source SSSP: row i of d_out (stride ld_out >= V) receives the
+ * distances from d_sources[i] to every vertex (column = vertex id).
+ * Strict relaxation from INF; rows are exact shortest-path distances. */
+int fap_sssp_multi(fap_ctx *ctx, const fap_graph_dev *g,
+                   const int32_t *d_sources, int32_t n_src, int32_t *d_out,
+                   int64_t ld_out, void *stream);
+
+/* In-place Floyd-Warshall closure of the n x n matrix at d_mat (stride ld),
+ * blocked: diagonal tile in LDS, row/column panels, remaining tiles. */
+int fap_fw_blocked(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t ld,
+                   void *stream);
+
+/* Min-plus product: C[i][j] = min(base, min_k A[i][k] + B[k][j]) where base
+ * is C[i][j] if accumulate else INF.  A is M x K (lda), B is K x N (ldb),
+ * C is M x N (ldc).  K may be 0 (C = base). */
+int fap_minplus(fap_ctx *ctx, const int32_t *A, int64_t lda, const int32_t *B,
+                int64_t ldb, int32_t *C, int64_t ldc, int32_t M, int32_t N,
+                int32_t K, int accumulate, void *stream);
+
+/* ---- part glue ---- */
+/* Dense part matrix sub (n_sub x n_sub, stride ld_sub) for the part whose
+ * vertices are st2ed[start .. start+n_sub), boundary first (n_bdy of them):
+ * rows < n_bdy gathered from rowblock (stride ld_rb, columns = vertex ids),
+ * other rows INF plus the part's internal arcs (last duplicate wins),
+ * diagonal 0.  Needs fap_graph_set_partition. */
+int fap_gather_submat(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
+                      int32_t n_sub, int32_t n_bdy, const int32_t *rowblock,
+                      int64_t ld_rb, int32_t *sub, int64_t ld_sub,
+                      void *stream);
+/* rowblock[i][st2ed[start+j]] = sub[i][j] for i < rows, j < n_sub. */
+int fap_scatter_submat(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
+                       int32_t rows, int32_t n_sub, const int32_t *sub,
+                       int64_t ld_sub, int32_t *rowblock, int64_t ld_rb,
+                       void *stream);
+
+/* One part end to end, device resident: rowblock (n_sub x V, stride ld_rb)
+ * receives the distances from every vertex of part `part_id` (1-based) in
+ * st2ed order to all V vertices.  start/n_sub/n_bdy are the part's offset,
+ * size and boundary count from preCondition.  Uses context workspace. */
+int fap_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
+                       int32_t n_sub, int32_t n_bdy, int32_t *rowblock,
+                       int64_t ld_rb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FAP_DEV_H_ */

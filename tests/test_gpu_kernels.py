@@ -1,0 +1,241 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU oracle.
+
+Integer work: every comparison is bit-exact.  Shapes cover ragged edges
+(M, N, K not multiples of the 128/16 tiles), K = 0, INF-heavy operands,
+accumulate mode, non-minimal and unaligned leading dimensions.
+"""
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fastapsp_amd import synth
+
+pytestmark = pytest.mark.gpu
+INF = O.INF
+
+
+def rand_mat(rows, cols, seed, lo=0, hi=1000, p_inf=0.2):
+    r = synth.uniform_int(synth.stream(seed, 1, rows * cols), lo, hi).reshape(rows, cols)
+    m = synth.uniform01(synth.stream(seed, 2, rows * cols)).reshape(rows, cols) < p_inf
+    r[m] = INF
+    return r.astype(np.int32)
+
+
+def padded(ctx, host, ld):
+    """Device copy of host (rows x cols) with row stride ld >= cols."""
+    rows, cols = host.shape
+    buf = np.full((rows, ld), -7, np.int32)
+    buf[:, :cols] = host
+    return ctx.to_device(buf)
+
+
+def unpad(dev, rows, cols):
+    return dev.get().reshape(rows, -1)[:, :cols]
+
+
+# ---------------- min-plus ----------------
+
+MP_SHAPES = [(1, 1, 1), (5, 7, 3), (128, 128, 16), (129, 257, 17), (64, 300, 2), (300, 1000, 124),
+             (257, 129, 250), (33, 4099, 61)]
+
+
+@pytest.mark.parametrize("M,N,K", MP_SHAPES)
+def test_minplus_matches_oracle(gpu_ctx, M, N, K):
+    A = rand_mat(M, K, M * 7 + K)
+    B = rand_mat(K, N, N * 3 + K)
+    dA, dB = gpu_ctx.to_device(A), gpu_ctx.to_device(B)
+    dC = gpu_ctx.alloc((M, N))
+    gpu_ctx.minplus(dA.ptr, K, dB.ptr, N, dC.ptr, N, M, N, K)
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dC.get(), O.min_plus(A, B))
+    for d in (dA, dB, dC):
+        d.free()
+
+
+def test_minplus_golden(gpu_ctx):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "minplus_13x37x29.npz"))
+    A, B = g["A"], g["B"]
+    dA, dB = gpu_ctx.to_device(A), gpu_ctx.to_device(B)
+    dC = gpu_ctx.alloc((13, 29))
+    gpu_ctx.minplus(dA.ptr, 37, dB.ptr, 29, dC.ptr, 29, 13, 29, 37)
+    np.testing.assert_array_equal(dC.get(), g["C"])
+
+
+@pytest.mark.parametrize("M,N,K,lda,ldb,ldc", [(70, 90, 33, 35, 93, 91), (128, 256, 32, 36, 260, 264),
+                                                (17, 130, 5, 5, 131, 133)])
+def test_minplus_strided_accumulate(gpu_ctx, M, N, K, lda, ldb, ldc):
+    A = rand_mat(M, K, 101)
+    B = rand_mat(K, N, 102)
+    C0 = rand_mat(M, N, 103, hi=3000, p_inf=0.3)
+    dA, dB, dC = padded(gpu_ctx, A, lda), padded(gpu_ctx, B, ldb), padded(gpu_ctx, C0, ldc)
+    gpu_ctx.minplus(dA.ptr, lda, dB.ptr, ldb, dC.ptr, ldc, M, N, K, accumulate=True)
+    got = dC.get().reshape(M, ldc)
+    np.testing.assert_array_equal(got[:, :N], O.min_plus_acc(A, B, C0))
+    assert (got[:, N:] == -7).all(), "wrote outside the row"
+
+
+def test_minplus_unaligned_pointers(gpu_ctx):
+    M, N, K = 50, 77, 19
+    A = rand_mat(M, K, 201)
+    B = rand_mat(K, N, 202)
+    dA = gpu_ctx.to_device(np.concatenate([[0], A.ravel()]))
+    dB = gpu_ctx.to_device(np.concatenate([[0, 0, 0], B.ravel()]))
+    dC = gpu_ctx.alloc(M * N + 1)
+    gpu_ctx.minplus(dA.offset(1), K, dB.offset(3), N, dC.offset(1), N, M, N, K)
+    np.testing.assert_array_equal(dC.get()[1:].reshape(M, N), O.min_plus(A, B))
+
+
+def test_minplus_k0_and_all_inf(gpu_ctx):
+    M, N = 40, 50
+    dC = gpu_ctx.to_device(np.zeros((M, N), np.int32))
+    gpu_ctx.minplus(0, 1, 0, N, dC.ptr, N, M, N, 0)
+    assert (dC.get() == INF).all()
+    A = np.full((M, 8), INF, np.int32)
+    B = np.full((8, N), INF, np.int32)
+    dA, dB = gpu_ctx.to_device(A), gpu_ctx.to_device(B)
+    gpu_ctx.minplus(dA.ptr, 8, dB.ptr, N, dC.ptr, N, M, N, 8)
+    assert (dC.get() == INF).all()
+
+
+def test_minplus_rejects_bad_arguments(gpu_ctx):
+    import fastapsp_amd as fa
+    dC = gpu_ctx.alloc((4, 4))
+    with pytest.raises(fa.FapError):
+        gpu_ctx.minplus(dC.ptr, 4, dC.ptr, 4, dC.ptr, 3, 4, 4, 4)  # ldc < N
+    with pytest.raises(fa.FapError):
+        gpu_ctx.minplus(dC.ptr, 4, dC.ptr, 4, dC.ptr, 4, -1, 4, 4)
+
+
+# ---------------- Floyd-Warshall ----------------
+
+@pytest.mark.parametrize("n", [1, 2, 5, 64, 127, 128, 129, 300, 513])
+def test_fw_matches_oracle(gpu_ctx, n):
+    m = synth.dense_matrix(n, min(1.0, 6.0 / n), 1, 1000, n)
+    d = gpu_ctx.to_device(m)
+    gpu_ctx.fw(d.ptr, n, n)
+    np.testing.assert_array_equal(d.get(), O.floyd(m))
+
+
+def test_fw_golden_and_strided(gpu_ctx):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dense70.npz"))
+    for ld in (70, 71, 128):
+        d = padded(gpu_ctx, g["mat"], ld)
+        gpu_ctx.fw(d.ptr, 70, ld)
+        got = d.get().reshape(70, ld)
+        np.testing.assert_array_equal(got[:, :70], g["closure"])
+        assert (got[:, 70:] == -7).all()
+
+
+def test_fw_positive_diagonal_and_zero_weights(gpu_ctx):
+    n = 200
+    m = synth.dense_matrix(n, 0.03, 0, 50, 77)
+    rng = np.random.default_rng(3)
+    m[np.arange(n), np.arange(n)] = rng.integers(0, 500, n)  # classical FW keeps min(d_ii, cycle)
+    d = gpu_ctx.to_device(m)
+    gpu_ctx.fw(d.ptr, n, n)
+    np.testing.assert_array_equal(d.get(), O.floyd(m))
+
+
+# ---------------- SSSP ----------------
+
+@pytest.mark.parametrize("name", ["grid8.npz", "rgg100.npz"])
+def test_sssp_golden_all_sources(gpu_ctx, name):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name))
+    ro, col, w = g["row_offset"], g["col"], g["wt"]
+    V = len(ro) - 1
+    dg = gpu_ctx.upload_graph(ro, col, w)
+    src = gpu_ctx.to_device(np.arange(V, dtype=np.int32))
+    out = gpu_ctx.alloc((V, V))
+    dg.sssp(src.ptr, V, out.ptr, V)
+    np.testing.assert_array_equal(out.get(), g["apsp"])
+    dg.close()
+
+
+def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx):
+    u, v, w = synth.grid_graph(64, 64, 0.05, 0.0, 1, 1, 1)
+    V = 4096
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    sources = np.array([0, 4095, 2080, 17, 17, 3000] + list(range(100, 4000, 97)), np.int32)
+    S, ld = len(sources), V + 5
+    dsrc = gpu_ctx.to_device(sources)
+    out = gpu_ctx.to_device(np.full((S, ld), -7, np.int32))
+    dg.sssp(dsrc.ptr, S, out.ptr, ld)
+    got = out.get()
+    np.testing.assert_array_equal(got[:, :V], O.batched_sssp(sources, ro, col, wt))
+    assert (got[:, V:] == -7).all()
+    dg.close()
+
+
+def test_sssp_directed_zero_weights(gpu_ctx):
+    au, av, aw = synth.rgg_graph(500, 5, 0, 9, 5)
+    ro, col, w = synth.to_csr(500, au, av, aw, True)
+    dg = gpu_ctx.upload_graph(ro, col, w)
+    sources = np.arange(0, 500, 3, dtype=np.int32)
+    dsrc = gpu_ctx.to_device(sources)
+    out = gpu_ctx.alloc((len(sources), 500))
+    dg.sssp(dsrc.ptr, len(sources), out.ptr, 500)
+    np.testing.assert_array_equal(out.get(), O.batched_sssp(sources, ro, col, w))
+    dg.close()
+
+
+def test_sssp_rejects_out_of_range_source(gpu_ctx):
+    import fastapsp_amd as fa
+    ro = np.array([0, 1, 2], np.int32)
+    dg = gpu_ctx.upload_graph(ro, np.array([1, 0], np.int32), np.array([3, 3], np.int32))
+    dsrc = gpu_ctx.to_device(np.array([0, 2], np.int32))
+    out = gpu_ctx.alloc((2, 2))
+    with pytest.raises(fa.FapError):
+        dg.sssp(dsrc.ptr, 2, out.ptr, 2)
+    dg.close()
+
+
+# ---------------- glue + one part end to end ----------------
+
+def partitioned_grid(R, K, seed, wl=1, wh=30):
+    u, v, w = synth.grid_graph(R, R, 0.1, 0.03, wl, wh, seed)
+    V = R * R
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    gid = (np.arange(V) * K // V + 1).astype(np.int32)
+    return V, ro, col, wt, gid, O.precondition(K, gid, ro, col)
+
+
+def test_gather_scatter_match_oracle(gpu_ctx):
+    V, ro, col, wt, gid, pc = partitioned_grid(24, 4, 9)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    for pid in (1, 2):
+        s, n, b = int(pc["ver_off"][pid]), int(pc["ver_num"][pid]), int(pc["bdy_num"][pid])
+        rb = O.batched_sssp(pc["st2ed"][s:s + b], ro, col, wt)
+        drb = gpu_ctx.to_device(rb)
+        ld_sub = n + 3
+        dsub = gpu_ctx.to_device(np.full((n, ld_sub), -7, np.int32))
+        dg.gather(s, n, b, drb.ptr, V, dsub.ptr, ld_sub)
+        sub = dsub.get().reshape(n, ld_sub)
+        want = O.submat_build(rb, gid, s, n, b, pc["st2ed"], pc["ed2st"], ro, col, wt)
+        np.testing.assert_array_equal(sub[:, :n], want)
+        assert (sub[:, n:] == -7).all()
+        blk0 = rand_mat(n, V, 55)
+        dblk = gpu_ctx.to_device(blk0)
+        dg.scatter(s, n, n, dsub.ptr, ld_sub, dblk.ptr, V)
+        np.testing.assert_array_equal(dblk.get(), O.submat_decode(want, blk0, s, pc["st2ed"]))
+    dg.close()
+
+
+@pytest.mark.parametrize("R,K", [(20, 4), (33, 7), (16, 1)])
+def test_subgraph_solve_matches_oracle(gpu_ctx, R, K):
+    V, ro, col, wt, gid, pc = partitioned_grid(R, K, R + K)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    for pid in range(1, K + 1):
+        s, n, b = int(pc["ver_off"][pid]), int(pc["ver_num"][pid]), int(pc["bdy_num"][pid])
+        out = gpu_ctx.alloc((n, V))
+        dg.subgraph_solve(s, n, b, out.ptr, V)
+        got = out.get()
+        np.testing.assert_array_equal(got, O.run_part(pid, ro, col, wt, gid, pc))
+        out.free()
+    dg.close()

@@ -1,0 +1,165 @@
+"""Host side of the drop-in, on the CPU (no GPU calls): the C ABI library
+loads and exports every declared symbol, the .mtx reader reproduces the
+reference's CSR order, preCondition reproduces findBoundry/sort_and_encode,
+the built-in partitioner is balanced and deterministic, and bad input is
+reported as an error code."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import fastapsp_amd as fa
+from fastapsp_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_GRAPH_DIR = "/root/reference/app"
+
+
+def declared_symbols():
+    names = set()
+    for h in ("fap_dev.h", "fap_graph.h", "fap_ops.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fap\w*)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fa.load()
+    syms = declared_symbols()
+    assert len(syms) > 50
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the Python bindings cover them
+    from fastapsp_amd import _SIGS
+    assert not (syms - set(_SIGS)), sorted(syms - set(_SIGS))
+
+
+def test_no_gpu_calls_report_errors_cleanly():
+    lib = fa.load()
+    assert lib.fap_version().startswith(b"fap-mi355x")
+    # null arguments are rejected before any HIP call
+    assert lib.fap_minplus(None, None, 1, None, 1, None, 1, 1, 1, 1, 0, None) == -1
+    assert "null" in fa.last_error()
+
+
+def csr_of(R, drop, diag, wl, wh, seed):
+    u, v, w = synth.grid_graph(R, R, drop, diag, wl, wh, seed)
+    V = R * R
+    return V, u, v, w, synth.to_csr(V, u, v, w, False)
+
+
+@pytest.mark.parametrize("weighted,directed", [(False, False), (True, False), (True, True),
+                                               (False, True)])
+def test_reader_matches_reference_order(tmp_path, monkeypatch, weighted, directed):
+    V, u, v, w, _ = csr_of(12, 0.1, 0.05, 1, 40, 3)
+    sub = ("weight-" if weighted else "unweight-") + ("directed" if directed else "undirected")
+    synth.write_mtx(str(tmp_path / "graph" / sub / "t.mtx"), V, u, v, w if weighted else None)
+    # a leading comment line, as the reader skips them
+    p = tmp_path / "graph" / sub / "t.mtx"
+    p.write_text("%%MatrixMarket test\n% comment\n" + p.read_text())
+    monkeypatch.chdir(tmp_path)
+    with fa.Graph.from_file("t", directed, weighted, 3) as g:
+        ro, col, wt = g.csr()
+    ww = w if weighted else np.ones_like(w)
+    e_ro, e_col, e_w = synth.csr_reference_order(V, u, v, ww, directed)
+    np.testing.assert_array_equal(ro, e_ro)
+    np.testing.assert_array_equal(col, e_col)
+    np.testing.assert_array_equal(wt, e_w)
+    if O.ref_available():
+        _, r_ro, r_col, r_w = O.ref_read("t", directed, weighted)
+        np.testing.assert_array_equal(ro, r_ro)
+        np.testing.assert_array_equal(col, r_col[: len(col)])
+        np.testing.assert_array_equal(wt, r_w[: len(col)].astype(np.int32))
+
+
+@pytest.mark.skipif(not (O.ref_available() and os.path.isdir(REF_GRAPH_DIR + "/graph")),
+                    reason="needs the reference's fixture graph and oracle/_ref")
+def test_reader_on_reference_fixture(monkeypatch):
+    monkeypatch.chdir(REF_GRAPH_DIR)
+    with fa.Graph.from_file("luxembourg_osm", False, False, 128) as g:
+        ro, col, wt = g.csr()
+        assert g.num_vertexs == 114599
+    _, r_ro, r_col, r_w = O.ref_read("luxembourg_osm", False, False)
+    np.testing.assert_array_equal(ro, r_ro)
+    np.testing.assert_array_equal(col, r_col)
+    assert (wt == 1).all() and (r_w == 1).all()
+
+
+@pytest.mark.parametrize("K", [1, 2, 7, 8])
+def test_precondition_matches_oracle_for_given_partition(K):
+    V, u, v, w, (ro, col, wt) = csr_of(18, 0.1, 0.03, 1, 9, K)
+    gid = np.random.default_rng(K).integers(1, K + 1, V).astype(np.int32)
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), K) as g:
+        g.setPartition(gid)
+        g.preCondition()
+        pc = O.precondition(K, gid, ro, col)
+        np.testing.assert_array_equal(g.getGraphId(), gid)
+        np.testing.assert_array_equal(g.getMapping(), pc["st2ed"])
+        np.testing.assert_array_equal(g.boundary_flags(), pc["is_bdy"])
+        vn, vo, bn, bo = g.part_info()
+        for a, b in ((vn, pc["ver_num"]), (vo, pc["ver_off"]), (bn, pc["bdy_num"]),
+                     (bo, pc["bdy_off"])):
+            np.testing.assert_array_equal(a[1:], b[1:])
+        assert g.isSplit() == (V > 20000)
+
+
+def test_partitioner_balanced_deterministic_and_compact():
+    V, u, v, w, (ro, col, wt) = csr_of(64, 0.05, 0.0, 1, 1, 1)
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), 8) as g1, \
+            fa.Graph.from_csr(ro, col, wt.astype(np.float32), 8) as g2:
+        a, b = g1.getGraphId(), g2.getGraphId()
+        np.testing.assert_array_equal(a, b)
+        assert g1.used_metis() is False
+        counts = np.bincount(a, minlength=9)[1:]
+        assert counts.min() >= 500 and counts.max() <= 524
+        g1.preCondition()
+        _, _, bn, _ = g1.part_info()
+        # a 64x64 grid cut 8 ways: a compact cut keeps boundaries near the perimeter
+        assert bn[1:].max() < 110
+
+
+@pytest.mark.parametrize("K", [3, 5, 16])
+def test_partitioner_handles_odd_K_and_components(K):
+    # two disconnected grids plus isolated vertices
+    V1, u1, v1, w1, _ = csr_of(10, 0.0, 0.0, 1, 1, 1)
+    u = np.concatenate([u1, u1 + V1])
+    v = np.concatenate([v1, v1 + V1])
+    V = 2 * V1 + 5
+    ro, col, wt = synth.to_csr(V, u, v, np.ones_like(u), False)
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), K) as g:
+        gid = g.getGraphId()
+        assert gid.min() >= 1 and gid.max() <= K
+        counts = np.bincount(gid, minlength=K + 1)[1:]
+        assert counts.sum() == V and counts.max() - counts.min() <= 2
+
+
+def test_bad_inputs_are_errors(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(fa.FapError) as e:
+        fa.Graph.from_file("missing", False, False, 2)
+    assert e.value.code == -5
+    d = tmp_path / "graph" / "weight-undirected"
+    d.mkdir(parents=True)
+    (d / "bad.mtx").write_text("3 3 2\n1 2 1.5\n2 3 2\n")
+    with pytest.raises(fa.FapError):
+        fa.Graph.from_file("bad", False, True, 2)   # non-integral weight
+    (d / "oob.mtx").write_text("3 3 2\n1 2 1\n2 9 2\n")
+    with pytest.raises(fa.FapError):
+        fa.Graph.from_file("oob", False, True, 2)   # vertex id out of range
+    (d / "ok.mtx").write_text("3 3 2\n1 2 1\n2 3 2\n")
+    with pytest.raises(fa.FapError):
+        fa.Graph.from_file("ok", False, True, 4)    # K > V
+    with pytest.raises(fa.FapError):
+        fa.Graph.from_csr(np.array([0, 1, 2]), np.array([1, 5]), np.array([1.0, 1.0]), 1)
+    g = fa.Graph.from_file("ok", False, True, 2)
+    with pytest.raises(fa.FapError):
+        g.setPartition(np.array([1, 2, 3], np.int32))  # id > K
+    with pytest.raises(fa.FapError):
+        g.solve()                                      # before preCondition
+    g.close()
