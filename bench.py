@@ -91,38 +91,36 @@ def bench_dense(args):
     dA, dB = ctx.to_device(A), ctx.to_device(B)
     dC = ctx.alloc((n, n))
     s = ctx.stream
-    ev = [C.c_void_p() for _ in range(4)]
+    nev = 4 * (args.steps + args.warmup)
+    ev = [C.c_void_p() for _ in range(nev)]
     for e in ev:
         fa.check(lib.fap_event_create(C.byref(e)))
 
-    def step(record=False):
-        fa.check(lib.fap_memcpy_d2d(ctx.handle, d_fw.ptr, d_src.ptr, d_fw.nbytes))
-        if record:
-            fa.check(lib.fap_event_record(ev[0], s))
+    def step(i):
+        fa.check(lib.fap_copy_i32(ctx.handle, d_fw.ptr, d_src.ptr, n * n, s))
+        fa.check(lib.fap_event_record(ev[4 * i], s))
         ctx.fw(d_fw.ptr, n, n, s)
-        if record:
-            fa.check(lib.fap_event_record(ev[1], s))
-            fa.check(lib.fap_event_record(ev[2], s))
+        fa.check(lib.fap_event_record(ev[4 * i + 1], s))
+        fa.check(lib.fap_event_record(ev[4 * i + 2], s))
         ctx.minplus(dA.ptr, n, dB.ptr, n, dC.ptr, n, n, n, n, False, s)
-        if record:
-            fa.check(lib.fap_event_record(ev[3], s))
+        fa.check(lib.fap_event_record(ev[4 * i + 3], s))
 
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    fw_ms, mp_ms = [], []
+    for i in range(args.warmup):
+        step(args.steps + i)
     ctx.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(record=True)
-        ms = C.c_float()
-        fa.check(lib.fap_event_elapsed(ev[0], ev[1], C.byref(ms)))
-        fw_ms.append(ms.value)
-        fa.check(lib.fap_event_elapsed(ev[2], ev[3], C.byref(ms)))
-        mp_ms.append(ms.value)
-        log(f"[bench] step {i}: fw {fw_ms[-1]:.2f} ms, min-plus {mp_ms[-1]:.2f} ms")
+        step(i)
     ctx.sync()
     total = time.perf_counter() - t0
+    fw_ms, mp_ms = [], []
+    for i in range(args.steps):
+        ms = C.c_float()
+        fa.check(lib.fap_event_elapsed(ev[4 * i], ev[4 * i + 1], C.byref(ms)))
+        fw_ms.append(ms.value)
+        fa.check(lib.fap_event_elapsed(ev[4 * i + 2], ev[4 * i + 3], C.byref(ms)))
+        mp_ms.append(ms.value)
+        log(f"[bench] step {i}: fw {fw_ms[-1]:.2f} ms, min-plus {mp_ms[-1]:.2f} ms")
     ops_step = 4.0 * n ** 3
     ms_step = 1e3 * total / args.steps
     value = ops_step / (total / args.steps) / 1e9
@@ -130,8 +128,10 @@ def bench_dense(args):
     fw_avg = float(np.mean(fw_ms))
     ach = 2.0 * n ** 3 / (mp_avg * 1e-3) / 1e12
     ach_fw = 2.0 * n ** 3 / (fw_avg * 1e-3) / 1e12
-    log("[bench] cpu baseline ...")
-    cpu = cpu_baseline_minplus(n, A, B, rows=args.cpu_rows)
+    cpu = None
+    if args.cpu_rows > 0:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline_minplus(n, A, B, rows=args.cpu_rows)
     res = {
         "metric": "min-plus GOp/s",
         "value": round(value, 1),
@@ -161,6 +161,38 @@ def bench_dense(args):
     return 0
 
 
+def bench_copy(args):
+    """Streaming copy of n*n int32 (known bytes): HBM reference and the
+    calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import fastapsp_amd as fa
+    lib = fa.load()
+    n = args.n
+    ctx = fa.Context(0)
+    a, b = ctx.alloc((n, n)), ctx.alloc((n, n))
+    ctx.fill(a, 1)
+    s = ctx.stream
+    ev = [C.c_void_p() for _ in range(2)]
+    for e in ev:
+        fa.check(lib.fap_event_create(C.byref(e)))
+    for _ in range(args.warmup):
+        fa.check(lib.fap_copy_i32(ctx.handle, b.ptr, a.ptr, n * n, s))
+    ctx.sync()
+    fa.check(lib.fap_event_record(ev[0], s))
+    for _ in range(args.steps):
+        fa.check(lib.fap_copy_i32(ctx.handle, b.ptr, a.ptr, n * n, s))
+    fa.check(lib.fap_event_record(ev[1], s))
+    ms = C.c_float()
+    fa.check(lib.fap_event_elapsed(ev[0], ev[1], C.byref(ms)))
+    per = ms.value / args.steps
+    gbs = 2 * 4 * n * n / (per * 1e-3) / 1e9
+    print(json.dumps({"metric": "copy GB/s", "value": round(gbs, 1), "unit": "GB/s",
+                      "bytes_per_launch": 8 * n * n, "ms_per_launch": round(per, 4),
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "higher_is_better": True, "dtype": "int32", "data": "synthetic",
+                      "config": {"workload": f"copy {n}x{n} int32"}}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,11 +200,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-rows", type=int, default=24)
+    ap.add_argument("--workload", default="dense", choices=["dense", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the parts of a "
               "graph shard across GPUs without exchange, see DESIGN.md", file=sys.stderr)
         return 2
+    if args.workload == "copy":
+        return bench_copy(args)
     return bench_dense(args)
 
 

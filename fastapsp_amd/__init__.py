@@ -44,6 +44,7 @@ _SIGS = {
     "fap_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
     "fap_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
     "fap_fill_i32": (_int, [_vp, _vp, _i64, _i32, _vp]),
+    "fap_copy_i32": (_int, [_vp, _vp, _vp, _i64, _vp]),
     "fap_checksum_i32": (_int, [_vp, _vp, _i64, _pu64, _pu64, _vp]),
     "fap_stream_create": (_int, [_vp, C.POINTER(_vp)]),
     "fap_stream_destroy": (_int, [_vp, _vp]),

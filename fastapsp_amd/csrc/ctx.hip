@@ -63,6 +63,23 @@ int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s) {
   return FAP_OK;
 }
 
+// Plain streaming copy (16-byte vectors when both pointers allow it); also
+// the known-size reference kernel for calibrating HBM byte counters.
+__global__ void copy_i32_kernel(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
+                                int64_t n, int vec) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vec) {
+    const int64_t n4 = n >> 2;
+    const int4 *s4 = reinterpret_cast<const int4 *>(src);
+    int4 *d4 = reinterpret_cast<int4 *>(dst);
+    for (int64_t i = t; i < n4; i += stride) d4[i] = s4[i];
+    for (int64_t i = (n4 << 2) + t; i < n; i += stride) dst[i] = src[i];
+  } else {
+    for (int64_t i = t; i < n; i += stride) dst[i] = src[i];
+  }
+}
+
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s) {
   return launch_fill_i32((int32_t *)d, n, (int32_t)value, s);
 }
@@ -217,6 +234,18 @@ int fap_memcpy_d2d(fap_ctx *ctx, void *dst, const void *src, size_t bytes) {
 int fap_fill_i32(fap_ctx *ctx, int32_t *d, int64_t n, int32_t value, void *stream) {
   if (!ctx || (!d && n > 0)) return fail(FAP_E_ARG, "null argument");
   return launch_fill_i32(d, n, value, pick_stream(ctx, stream));
+}
+
+int fap_copy_i32(fap_ctx *ctx, int32_t *dst, const int32_t *src, int64_t n, void *stream) {
+  if (!ctx || ((!dst || !src) && n > 0)) return fail(FAP_E_ARG, "null argument");
+  if (n <= 0) return FAP_OK;
+  const int vec = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+  const int64_t work = vec ? (n + 3) / 4 : n;
+  const int64_t blocks = std::min<int64_t>((work + 255) / 256, (int64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(copy_i32_kernel, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0,
+                     pick_stream(ctx, stream), src, dst, n, vec);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
 }
 
 int fap_checksum_i32(fap_ctx *ctx, const int32_t *d, int64_t n, uint64_t *sum,

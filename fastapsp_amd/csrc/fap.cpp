@@ -50,6 +50,11 @@ struct fapGraph::Device {
   }
 };
 
+void fapGraph::reset_status() {
+  status_ = 0;
+  error_.clear();
+}
+
 int32_t fapGraph::fail(int32_t code, const std::string &msg) {
   status_ = code;
   error_ = msg;
@@ -202,6 +207,7 @@ int32_t fapGraph::setPartition(const std::vector<int32_t> &gid) {
 
 // fap/fap.cpp:67-81
 void fapGraph::preCondition() {
+  reset_status();
   if (!loaded_) {
     fail(FAP_E_STATE, "graph not loaded: " + error_);
     return;
@@ -290,6 +296,7 @@ int32_t fapGraph::run(int32_t id, int32_t *d_rowblock) {
 
 // fap/fap.cpp:184-194.  The full V x V matrix, parts stacked in part order.
 int32_t fapGraph::solve(bool is_path_needed_) {
+  reset_status();
   is_path_needed = is_path_needed_;
   if (!preconditioned_) return fail(FAP_E_STATE, "solve() before preCondition()");
   if (ensure_device() != 0) return -1;
@@ -320,6 +327,7 @@ int32_t fapGraph::solve(bool is_path_needed_) {
 
 // fap/fap.cpp:197-207
 int32_t fapGraph::solveSubGraph(int32_t id, bool is_path_needed_) {
+  reset_status();
   is_path_needed = is_path_needed_;
   if (!part_ok(id)) return fail(FAP_E_ARG, "part id outside 1..K or preCondition() not run");
   if (ensure_device() != 0) return -1;
@@ -350,6 +358,7 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool is_path_needed_) {
 std::vector<int32_t> fapGraph::getMapping() { return st2ed; }
 
 std::vector<int32_t> fapGraph::getAllDistanceInt32() {
+  reset_status();
   if (!full_solved_ || !dev_) {
     fail(FAP_E_STATE, "solve() has not produced the full matrix");
     return {};
@@ -397,6 +406,7 @@ const int32_t *fapGraph::subGraphDistanceDevice(int32_t id, int64_t *ld) {
 }
 
 std::vector<int32_t> fapGraph::getSubGraphDistanceInt32(int32_t id) {
+  reset_status();
   int64_t ld = 0;
   const int32_t *p = subGraphDistanceDevice(id, &ld);
   if (!p) {
@@ -432,6 +442,7 @@ std::vector<int32_t> fapGraph::getSubGraphPath(int32_t) {
 }
 
 std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
+  reset_status();
   if (v < 0 || v >= num_vertexs || !part_ok(current_subgraph_id) ||
       graph_id[v] != current_subgraph_id) {
     fail(FAP_E_ARG, "vertex is not in the current part");
@@ -466,6 +477,7 @@ float fapGraph::getDistanceP2P() {
 }
 
 int32_t fapGraph::getDistanceP2P(int32_t src, int32_t dst, int32_t *out) {
+  reset_status();
   if (!out || src < 0 || src >= num_vertexs || dst < 0 || dst >= num_vertexs)
     return fail(FAP_E_ARG, "bad vertex id");
   if (!preconditioned_) return fail(FAP_E_STATE, "preCondition() not run");
@@ -485,6 +497,7 @@ std::vector<int32_t> fapGraph::getPathP2P() {
 }
 
 int32_t fapGraph::subGraphChecksum(int32_t id, uint64_t *sum, uint64_t *weighted_sum) {
+  reset_status();
   int64_t ld = 0;
   const int32_t *p = subGraphDistanceDevice(id, &ld);
   if (!p) return fail(FAP_E_STATE, "part not solved");

@@ -74,6 +74,7 @@ class fapGraph {
   std::string error_;
 
   int32_t fail(int32_t code, const std::string &msg);
+  void reset_status();
   int32_t ensure_device();
   int32_t run(int32_t subgraph_id, int32_t *d_rowblock);
   void init_from_csr(int32_t V, std::vector<int32_t> &&ro, std::vector<int32_t> &&col,

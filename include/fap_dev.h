@@ -69,6 +69,8 @@ int fap_memcpy_d2h(fap_ctx *ctx, void *dst, const void *src, size_t bytes);
 int fap_memcpy_d2d(fap_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Fill n int32 with value (a kernel, not hipMemset; asynchronous). */
 int fap_fill_i32(fap_ctx *ctx, int32_t *d, int64_t n, int32_t value, void *stream);
+/* Device copy of n int32 by a streaming kernel (asynchronous). */
+int fap_copy_i32(fap_ctx *ctx, int32_t *dst, const int32_t *src, int64_t n, void *stream);
 /* Wrapping sum of n uint32 and of (index+1)*value (a position-keyed checksum);
  * synchronous. */
 int fap_checksum_i32(fap_ctx *ctx, const int32_t *d, int64_t n, uint64_t *sum,
