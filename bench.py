@@ -193,6 +193,275 @@ def bench_copy(args):
     return 0
 
 
+def bench_grid_pipeline(args):
+    """Full pipeline, plain form: 512x512 road-like grid (10% of edges removed,
+    2% of V diagonal shortcuts, w 1..100), K=256, parts solved one after the
+    other through the device stages (SSSP -> gather -> FW -> min-plus ->
+    scatter), each 1024 x 262144 row block checksummed on the device."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    lib = fa.load()
+    R, K = args.grid, args.parts_k
+    V = R * R
+    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    E = len(col)
+    g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
+    g.preCondition()
+    vn, vo, bn, _ = g.part_info()
+    st2ed = g.getMapping()
+    gid = g.getGraphId()
+    ed2st = np.empty(V, np.int32)
+    ed2st[st2ed] = np.arange(V, dtype=np.int32)
+    g.close()
+    ctx = fa.Context(0)
+    dg = ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, st2ed, ed2st)
+    nmax = int(vn[1:].max())
+    blk = ctx.alloc((nmax, V))
+    ld_sub = (nmax + 3) // 4 * 4
+    sub = ctx.alloc((nmax, ld_sub))
+    d_st2ed = ctx.to_device(st2ed)
+    s = ctx.stream
+    parts = list(range(1, K + 1))[: args.max_parts] if args.max_parts else list(range(1, K + 1))
+    nev = 5
+    ev = [C.c_void_p() for _ in range(nev)]
+    for e in ev:
+        fa.check(lib.fap_event_create(C.byref(e)))
+    t_sssp = t_fw = t_mp = t_glue = 0.0
+    ops_fw = ops_mp = 0.0
+    sums = []
+    ms = C.c_float()
+
+    def part(pid):
+        st, n, b = int(vo[pid]), int(vn[pid]), int(bn[pid])
+        ldsb = (n + 3) // 4 * 4
+        fa.check(lib.fap_event_record(ev[0], s))
+        dg.sssp(d_st2ed.offset(st), b, blk.ptr, V, s)
+        fa.check(lib.fap_event_record(ev[1], s))
+        dg.gather(st, n, b, blk.ptr, V, sub.ptr, ldsb, s)
+        ctx.fw(sub.ptr, n, ldsb, s)
+        fa.check(lib.fap_event_record(ev[2], s))
+        ctx.minplus(sub.offset(b * ldsb), ldsb, blk.ptr, V, blk.offset(b * V), V, n - b, V, b,
+                    False, s)
+        fa.check(lib.fap_event_record(ev[3], s))
+        dg.scatter(st, n, n, sub.ptr, ldsb, blk.ptr, V, s)
+        fa.check(lib.fap_event_record(ev[4], s))
+        return n, b
+
+    part(parts[0])  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    for pid in parts:
+        n, b = part(pid)
+        sums.append(ctx.checksum(blk.ptr, n * V))  # synchronizes
+        fa.check(lib.fap_event_elapsed(ev[0], ev[1], C.byref(ms)))
+        t_sssp += ms.value
+        fa.check(lib.fap_event_elapsed(ev[1], ev[2], C.byref(ms)))
+        t_fw += ms.value
+        fa.check(lib.fap_event_elapsed(ev[2], ev[3], C.byref(ms)))
+        t_mp += ms.value
+        fa.check(lib.fap_event_elapsed(ev[3], ev[4], C.byref(ms)))
+        t_glue += ms.value
+        ops_fw += 2.0 * n ** 3
+        ops_mp += 2.0 * (n - b) * b * V
+    total = time.perf_counter() - t0
+    P = len(parts)
+    srcs = float(sum(int(bn[p]) for p in parts))
+    sssp_bytes = P * (4.0 * (V + 1) + 8.0 * E) + 4.0 * srcs * V
+    res = {
+        "metric": "min-plus GOp/s", "value": round((ops_fw + ops_mp) / total / 1e9, 1),
+        "unit": "GOp/s", "n_gpus": 1, "steps": P, "warmup": 1,
+        "ms_per_step": round(1e3 * total / P, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{R}x{R} grid (-10% edges, +2% diagonals, w 1..100), K={K}, "
+                               f"plain part-by-part solve", "V": V, "E": E, "K": K,
+                   "parts_timed": P, "mean_part": float(vn[1:].mean()),
+                   "mean_boundary": float(bn[1:].mean())},
+        "stage_ms": {"sssp": round(t_sssp, 2), "gather+fw": round(t_fw, 2),
+                     "minplus": round(t_mp, 2), "scatter": round(t_glue, 2),
+                     "wall_incl_checksum": round(total * 1e3, 2)},
+        "minplus_stage": {"ops": ops_mp, "TOP/s": round(ops_mp / (t_mp * 1e-3) / 1e12, 2),
+                          "frac_valu_peak": round(ops_mp / (t_mp * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
+        "fw_stage": {"ops": ops_fw, "TOP/s": round(ops_fw / (t_fw * 1e-3) / 1e12, 2)},
+        "sssp_stage": {"sources": srcs, "edges_per_s": srcs * E / (t_sssp * 1e-3),
+                       "algorithmic_GB_per_s": round(sssp_bytes / (t_sssp * 1e-3) / 1e9, 1)},
+        "checksum_first_parts": [list(map(int, x)) for x in sums[:3]],
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def _time_ms(lib, ctx, fn, reps=2):
+    """Average device milliseconds of fn() on the context stream."""
+    import fastapsp_amd as fa
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    fa.check(lib.fap_event_create(C.byref(e0)))
+    fa.check(lib.fap_event_create(C.byref(e1)))
+    fn()
+    ctx.sync()
+    fa.check(lib.fap_event_record(e0, ctx.stream))
+    for _ in range(reps):
+        fn()
+    fa.check(lib.fap_event_record(e1, ctx.stream))
+    ms = C.c_float()
+    fa.check(lib.fap_event_elapsed(e0, e1, C.byref(ms)))
+    lib.fap_event_destroy(e0)
+    lib.fap_event_destroy(e1)
+    return ms.value / reps
+
+
+def bench_fw_sweep(args):
+    """FW closure time per (n, block size, look-ahead): the tuning table."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    lib = fa.load()
+    ctx = fa.Context(0)
+    rows = []
+    for n in [int(x) for x in args.sizes.split(",")]:
+        m = synth.dense_matrix(n, min(1.0, 6.0 / n) if n < 600 else 0.01, 1, 1000, n)
+        src = ctx.to_device(m)
+        d = ctx.alloc((n, n))
+        for tile in (64, 128):
+            for look in (0, 1):
+                ctx.set_option("fw_tile", tile)
+                ctx.set_option("fw_lookahead", look)
+
+                def run():
+                    fa.check(lib.fap_copy_i32(ctx.handle, d.ptr, src.ptr, n * n, ctx.stream))
+                    ctx.fw(d.ptr, n, n, ctx.stream)
+                copy_ms = _time_ms(lib, ctx, lambda: fa.check(
+                    lib.fap_copy_i32(ctx.handle, d.ptr, src.ptr, n * n, ctx.stream)))
+                ms = _time_ms(lib, ctx, run, reps=args.steps) - copy_ms
+                r = {"n": n, "tile": tile, "lookahead": look, "ms": round(ms, 4),
+                     "TOP/s": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 3)}
+                rows.append(r)
+                print(json.dumps(r), flush=True)
+        src.free()
+        d.free()
+    ctx.set_option("fw_tile", 0)
+    ctx.set_option("fw_lookahead", 0)
+    return 0
+
+
+def bench_sssp_sweep(args):
+    """Multi-source SSSP time per (threads, delta) on one part's boundary of
+    the 512x512 grid (w 1..100) and of the 64x64 unit grid."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    lib = fa.load()
+    ctx = fa.Context(0)
+    for (R, drop, diag, wl, wh, seed, K) in [(512, 0.10, 0.02, 1, 100, 3, 256),
+                                             (64, 0.05, 0.0, 1, 1, 1, 8)]:
+        V = R * R
+        u, v, w = synth.grid_graph(R, R, drop, diag, wl, wh, seed)
+        ro, col, wt = synth.to_csr(V, u, v, w, False)
+        g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
+        g.preCondition()
+        vn, vo, bn, _ = g.part_info()
+        st2ed = g.getMapping()
+        g.close()
+        srcs = st2ed[int(vo[1]): int(vo[1]) + int(bn[1])]
+        dg = ctx.upload_graph(ro, col, wt)
+        dsrc = ctx.to_device(srcs)
+        out = ctx.alloc((len(srcs), V))
+        mean_w = float(wt.mean())
+        for threads in (256, 512, 1024):
+            for mult in (1, 2, 4, 8, 32):
+                delta = max(1, int(round(mult * mean_w)))
+                ctx.set_option("sssp_threads", threads)
+                ctx.set_option("sssp_delta", delta)
+                ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V,
+                                                        ctx.stream), reps=1)
+                r = {"graph": f"{R}x{R} w{wl}..{wh}", "sources": len(srcs), "threads": threads,
+                     "delta": delta, "ms": round(ms, 3),
+                     "edges_per_s": len(srcs) * len(col) / (ms * 1e-3)}
+                print(json.dumps(r), flush=True)
+        dg.close()
+    ctx.set_option("sssp_threads", 1024)
+    ctx.set_option("sssp_delta", 0)
+    return 0
+
+
+def bench_batch(args):
+    """Uneven batch: 512 random geometric graphs, n log-uniform in [64, 2048]
+    (points in the unit square, radius for average degree 6, both arcs with
+    independent weights 1..255), each closed to full APSP (K = 1: dense build
+    + blocked FW).  Plain = one graph per launch sequence, looped on the host
+    (the reference's structure); batched = all graphs in one persistent
+    launch fed by a size-ordered task queue."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    lib = fa.load()
+    ctx = fa.Context(0)
+    sizes = synth.loguniform_sizes(args.jobs, 64, 2048, 4).astype(int)
+    graphs, ptrs, lds = [], [], []
+    ld_of = [(n + 3) // 4 * 4 for n in sizes]
+    total = sum(n * ld for n, ld in zip(sizes, ld_of))
+    pool = ctx.alloc(total)
+    off = 0
+    for j, n in enumerate(sizes):
+        au, av, aw = synth.rgg_graph(int(n), 6, 1, 255, 1000 + j)
+        ro, col, w = synth.to_csr(int(n), au, av, aw, True)
+        graphs.append(ctx.upload_graph(ro, col, w))
+        ptrs.append(pool.offset(off))
+        lds.append(ld_of[j])
+        off += n * ld_of[j]
+    ops = float(sum(2.0 * n ** 3 for n in sizes))
+
+    def build():
+        ctx.dense_from_graphs(graphs, ptrs, lds)
+
+    def plain():
+        for p_, n, ld in zip(ptrs, sizes, lds):
+            ctx.fw(p_, int(n), int(ld), ctx.stream)
+
+    plan = ctx.fw_plan(ptrs, [int(x) for x in sizes], lds, use_graph=True)
+    info = plan.info()
+
+    def batched():
+        plan.run(ctx.stream)
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        build()
+        ctx.sync()
+        t_ms = []
+        for _ in range(args.steps):
+            build()
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            t_ms.append(1e3 * (time.perf_counter() - t0))
+        res[name] = float(np.median(t_ms))
+        res[name + "_sum"] = ctx.checksum(pool.ptr, total)
+        log(f"[bench] {name}: {res[name]:.2f} ms")
+    same = res["plain_sum"] == res["batched_sum"]
+    out = {
+        "metric": "min-plus GOp/s", "value": round(ops / (res["batched"] * 1e-3) / 1e9, 1),
+        "unit": "GOp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(res["batched"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{args.jobs} random geometric graphs, n log-uniform in "
+                               f"[64, 2048], deg 6, w 1..255, K=1 closure",
+                   "jobs": int(args.jobs), "sum_n": int(sizes.sum()), "ops": ops},
+        "plain": {"ms": round(res["plain"], 3),
+                  "GOp/s": round(ops / (res["plain"] * 1e-3) / 1e9, 1)},
+        "batched": {"ms": round(res["batched"], 3),
+                    "GOp/s": round(ops / (res["batched"] * 1e-3) / 1e9, 1),
+                    "frac_valu_peak": round(ops / (res["batched"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
+        "batched_launches": info["launches"], "batched_tasks": info["tasks"],
+        "speedup_batched_vs_plain": round(res["plain"] / res["batched"], 2),
+        "plain_and_batched_identical": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    for g in graphs:
+        g.close()
+    return 0 if same else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,7 +469,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-rows", type=int, default=24)
-    ap.add_argument("--workload", default="dense", choices=["dense", "copy"])
+    ap.add_argument("--workload", default="dense",
+                    choices=["dense", "copy", "grid", "fwsweep", "ssspsweep", "batch"])
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--sizes", default="256,512,1024,2048,4096,8192")
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--parts-k", type=int, default=256)
+    ap.add_argument("--max-parts", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the parts of a "
@@ -208,6 +483,14 @@ def main():
         return 2
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "grid":
+        return bench_grid_pipeline(args)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "fwsweep":
+        return bench_fw_sweep(args)
+    if args.workload == "ssspsweep":
+        return bench_sssp_sweep(args)
     return bench_dense(args)
 
 

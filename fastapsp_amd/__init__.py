@@ -37,6 +37,8 @@ _SIGS = {
     "fap_ctx_create": (_int, [_int, C.POINTER(_vp)]),
     "fap_ctx_destroy": (_int, [_vp]),
     "fap_ctx_stream": (_int, [_vp, C.POINTER(_vp)]),
+    "fap_ctx_set_option": (_int, [_vp, C.c_char_p, _i64]),
+    "fap_ctx_get_option": (_int, [_vp, C.c_char_p, C.POINTER(_i64)]),
     "fap_ctx_sync": (_int, [_vp]),
     "fap_dev_alloc": (_int, [_vp, _sz, C.POINTER(_vp)]),
     "fap_dev_free": (_int, [_vp, _vp]),
@@ -56,9 +58,18 @@ _SIGS = {
     "fap_graph_upload": (_int, [_vp, _i32, _i64, _pi32, _pi32, _pi32, C.POINTER(_vp)]),
     "fap_graph_set_partition": (_int, [_vp, _vp, _pi32, _pi32, _pi32]),
     "fap_graph_free": (_int, [_vp, _vp]),
+    "fap_graph_set_delta": (_int, [_vp, _i32]),
     "fap_graph_info": (_int, [_vp, _pi32, C.POINTER(_i64)]),
     "fap_sssp_multi": (_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp]),
     "fap_fw_blocked": (_int, [_vp, _vp, _i32, _i64, _vp]),
+    "fap_fw_batched": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _vp]),
+    "fap_fw_plan_create": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _int,
+                                  C.POINTER(_vp)]),
+    "fap_fw_plan_run": (_int, [_vp, _vp]),
+    "fap_fw_plan_info": (_int, [_vp, _pi32, C.POINTER(_i64), C.POINTER(_i64)]),
+    "fap_fw_plan_destroy": (_int, [_vp]),
+    "fap_dense_from_graphs": (_int, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i64),
+                                     _vp]),
     "fap_minplus": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _int, _vp]),
     "fap_gather_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fap_scatter_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
@@ -205,6 +216,14 @@ class Context:
     def sync(self):
         check(load().fap_ctx_sync(self.handle), "sync")
 
+    def set_option(self, name: str, value: int):
+        check(load().fap_ctx_set_option(self.handle, name.encode(), int(value)), "set_option")
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        check(load().fap_ctx_get_option(self.handle, name.encode(), C.byref(v)), "get_option")
+        return v.value
+
     def alloc(self, shape) -> DeviceArray:
         shape = (shape,) if isinstance(shape, (int, np.integer)) else tuple(shape)
         n = int(np.prod(shape)) if shape else 1
@@ -236,8 +255,55 @@ class Context:
     def fw(self, ptr: int, n: int, ld: int, stream=None):
         check(load().fap_fw_blocked(self.handle, ptr, n, ld, stream), "fap_fw_blocked")
 
+    def fw_batched(self, ptrs, ns, lds, stream=None):
+        k = len(ptrs)
+        P = (_vp * k)(*ptrs)
+        N = np.ascontiguousarray(ns, dtype=np.int32)
+        L = np.ascontiguousarray(lds, dtype=np.int64)
+        check(load().fap_fw_batched(self.handle, k, P, _p32(N), L.ctypes.data_as(C.POINTER(_i64)),
+                                    stream), "fap_fw_batched")
+
+    def fw_plan(self, ptrs, ns, lds, use_graph=True):
+        k = len(ptrs)
+        P = (_vp * k)(*ptrs)
+        N = np.ascontiguousarray(ns, dtype=np.int32)
+        L = np.ascontiguousarray(lds, dtype=np.int64)
+        h = _vp()
+        check(load().fap_fw_plan_create(self.handle, k, P, _p32(N),
+                                        L.ctypes.data_as(C.POINTER(_i64)), int(use_graph),
+                                        C.byref(h)), "fap_fw_plan_create")
+        return FwPlan(h.value)
+
+    def dense_from_graphs(self, graphs, ptrs, lds, stream=None):
+        k = len(graphs)
+        G = (_vp * k)(*[g.handle for g in graphs])
+        P = (_vp * k)(*ptrs)
+        L = np.ascontiguousarray(lds, dtype=np.int64)
+        check(load().fap_dense_from_graphs(self.handle, k, G, P, L.ctypes.data_as(C.POINTER(_i64)),
+                                           stream), "fap_dense_from_graphs")
+
     def upload_graph(self, row_offset, col, w) -> "DeviceGraph":
         return DeviceGraph(self, row_offset, col, w)
+
+
+class FwPlan:
+    """A reusable batched closure (fap_fw_plan): run() replays it."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def run(self, stream=None):
+        check(load().fap_fw_plan_run(self.handle, stream), "fap_fw_plan_run")
+
+    def info(self):
+        a, b, c = C.c_int32(), C.c_int64(), C.c_int64()
+        check(load().fap_fw_plan_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return {"launches": a.value, "tasks": b.value, "ops": c.value}
+
+    def close(self):
+        if self.handle:
+            load().fap_fw_plan_destroy(self.handle)
+            self.handle = None
 
 
 class DeviceGraph:
@@ -261,6 +327,9 @@ class DeviceGraph:
         e = np.ascontiguousarray(ed2st, dtype=np.int32)
         check(load().fap_graph_set_partition(self.ctx.handle, self.handle, _p32(g), _p32(s),
                                              _p32(e)), "fap_graph_set_partition")
+
+    def set_delta(self, delta: int):
+        check(load().fap_graph_set_delta(self.handle, int(delta)), "fap_graph_set_delta")
 
     def sssp(self, d_sources: int, n_src: int, d_out: int, ld_out: int, stream=None):
         check(load().fap_sssp_multi(self.ctx.handle, self.handle, d_sources, n_src, d_out,

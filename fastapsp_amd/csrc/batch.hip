@@ -1,0 +1,315 @@
+// batch.hip -- many uneven closures, phase-synchronous launches.
+//
+// The reference solves parts one after another (fap/fap.cpp:189-193) and
+// spreads them over processes with MPI (fap/fap_more_task_mpi.cpp:34-91).
+// Here a batch of matrices (the K parts of one graph, or independent graphs)
+// is closed together:
+//
+//  * each matrix (job) is cut into 64 x 64 tiles and its blocked
+//    Floyd-Warshall becomes a sequence of phases -- diag(r), panels(r),
+//    rest(r) for r = 0 .. nb-1 -- each phase a set of independent tasks;
+//  * phase p of every job goes into launch p: a queue of tasks, largest
+//    jobs first, consumed by a persistent grid (workgroup w takes tasks w,
+//    w + grid, ...).  Tasks of one launch never depend on each other; the
+//    kernel boundary orders phase p after phase p-1;
+//  * the launch sequence of a batch is a plan: descriptors uploaded once,
+//    launches captured in a hipGraph and replayed.
+//
+// There is no wait inside a kernel: every workgroup runs its tasks and
+// exits.  (A single-launch variant whose workgroups waited on per-job
+// completion counters made no progress on the device; it was taken out.)
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+#include "minplus_tile.h"
+
+namespace fapk {
+
+struct BatchJob {
+  int32_t *mat;
+  int64_t ld;
+  int32_t n;
+  int32_t nb;
+};
+
+enum : int { TASK_DIAG = 0, TASK_PANEL = 1, TASK_REST = 2 };
+
+constexpr int BT = 64;  // tile
+constexpr int BT_LD = BT + 4;
+constexpr int BATCH_LDS_WORDS =
+    (MpSmall::LDS_WORDS > BT * BT_LD) ? MpSmall::LDS_WORDS : BT * BT_LD;
+
+// Closure of one diagonal 64-block with 256 threads: thread t owns the
+// 1 x 4 strips at rows (t >> 4) + 16q, q < 4, columns (t & 15) * 4.
+__device__ __forceinline__ void batch_diag(int32_t *blk, int64_t ld, int kb, int32_t *D) {
+  const int t = threadIdx.x;
+  for (int idx = t; idx < BT * BT; idx += MP_THREADS) {
+    const int i = idx >> 6, j = idx & 63;
+    D[i * BT_LD + j] = (i < kb && j < kb) ? blk[(int64_t)i * ld + j] : kInf;
+  }
+  __syncthreads();
+  const int r0 = t >> 4, j = (t & 15) << 2;
+  for (int k = 0; k < kb; k++) {
+    const int4 b = *reinterpret_cast<const int4 *>(D + k * BT_LD + j);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = r0 + 16 * q;
+      const int a = D[i * BT_LD + k];
+      int4 *cp = reinterpret_cast<int4 *>(D + i * BT_LD + j);
+      int4 c = *cp;
+      c.x = imin(c.x, a + b.x);
+      c.y = imin(c.y, a + b.y);
+      c.z = imin(c.z, a + b.z);
+      c.w = imin(c.w, a + b.w);
+      *cp = c;
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < BT * BT; idx += MP_THREADS) {
+    const int i = idx >> 6, j2 = idx & 63;
+    if (i < kb && j2 < kb) blk[(int64_t)i * ld + j2] = D[i * BT_LD + j2];
+  }
+}
+
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_phase_kernel(
+    const BatchJob *__restrict__ jobs, const int4 *__restrict__ tasks, int ntasks) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[BATCH_LDS_WORDS];
+  for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const int4 task = tasks[ti];
+    const BatchJob job = jobs[task.x];
+    const int kind = task.y & 3, r = task.y >> 2, idx = task.z;
+    const int nb = job.nb, n = job.n;
+    const int64_t ld = job.ld;
+    int32_t *d = job.mat;
+    const int base = r * BT;
+    const int kb = imin(BT, n - base);
+    const int flags =
+        ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
+    if (kind == TASK_DIAG) {
+      batch_diag(d + (int64_t)base * ld + base, ld, kb, lds);
+    } else if (kind == TASK_PANEL) {
+      const int32_t *diag = d + (int64_t)base * ld + base;
+      if (idx < nb - 1) {
+        const int J = idx < r ? idx : idx + 1;
+        int32_t *c = d + (int64_t)base * ld + (int64_t)J * BT;
+        MpSmall::run(diag, ld, c, ld, c, ld, kb, imin(BT, n - J * BT), kb, true, flags, lds);
+      } else {
+        const int u = idx - (nb - 1);
+        const int I = u < r ? u : u + 1;
+        int32_t *c = d + (int64_t)I * BT * ld + base;
+        MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
+      }
+    } else {
+      const int m = nb - 1;
+      int I = idx / m, J = idx - (idx / m) * m;
+      I += (I >= r);
+      J += (J >= r);
+      int32_t *c = d + (int64_t)I * BT * ld + (int64_t)J * BT;
+      MpSmall::run(d + (int64_t)I * BT * ld + base, ld, d + (int64_t)base * ld + (int64_t)J * BT,
+                   ld, c, ld, imin(BT, n - I * BT), imin(BT, n - J * BT), kb, true, flags, lds);
+    }
+    __syncthreads();  // the LDS buffer is reused by the next task
+  }
+}
+
+}  // namespace fapk
+
+namespace fapi {
+
+// Tasks of a batch grouped by phase: group p holds phase p of every job that
+// has one, jobs in decreasing size (longest processing time first).  A task
+// is {job, kind | r << 2, index, 0}; a one-block job has only its diagonal.
+void build_fw_phases(const std::vector<int32_t> &n, std::vector<int4> *tasks,
+                     std::vector<int32_t> *group_off) {
+  const int J = (int)n.size();
+  std::vector<int32_t> order(J);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+  int max_phases = 0;
+  for (int j = 0; j < J; j++) {
+    const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
+    max_phases = std::max(max_phases, nb > 1 ? 3 * nb : (n[j] > 0 ? 1 : 0));
+  }
+  tasks->clear();
+  group_off->assign(1, 0);
+  for (int p = 0; p < max_phases; p++) {
+    for (int j : order) {
+      if (n[j] <= 0) continue;
+      const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
+      const int phases = nb > 1 ? 3 * nb : 1;
+      if (p >= phases) continue;
+      const int r = nb > 1 ? p / 3 : 0;
+      const int kind = nb > 1 ? p % 3 : fapk::TASK_DIAG;
+      const int count = kind == fapk::TASK_DIAG    ? 1
+                        : kind == fapk::TASK_PANEL ? 2 * (nb - 1)
+                                                   : (nb - 1) * (nb - 1);
+      for (int i = 0; i < count; i++) tasks->push_back(make_int4(j, kind | (r << 2), i, 0));
+    }
+    group_off->push_back((int32_t)tasks->size());
+  }
+}
+
+}  // namespace fapi
+
+struct fap_fw_plan {
+  fap_ctx *ctx = nullptr;
+  void *desc = nullptr;        // jobs | tasks (device)
+  int njobs = 0;
+  size_t tasks_off = 0;
+  std::vector<int32_t> group_off;
+  int64_t ops = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace fapi {
+
+static int plan_launch(fap_fw_plan *p, hipStream_t s) {
+  const fapk::BatchJob *jobs = (const fapk::BatchJob *)p->desc;
+  const int4 *tasks = (const int4 *)((char *)p->desc + p->tasks_off);
+  const int cap = p->ctx->num_cus * 4;
+  for (size_t g = 0; g + 1 < p->group_off.size(); g++) {
+    const int cnt = p->group_off[g + 1] - p->group_off[g];
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(fapk::fw_phase_kernel, dim3(std::min(cnt, cap)), dim3(fapk::MP_THREADS),
+                       0, s, jobs, tasks + p->group_off[g], cnt);
+    FAP_CHECK_LAUNCH();
+  }
+  return FAP_OK;
+}
+
+int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int32_t *n,
+                   const int64_t *ld, int use_graph, fap_fw_plan **out) {
+  std::vector<int32_t> sizes(n, n + count);
+  std::vector<int4> tasks;
+  std::vector<int32_t> goff;
+  build_fw_phases(sizes, &tasks, &goff);
+  std::vector<fapk::BatchJob> jobs(count);
+  int64_t ops = 0;
+  for (int j = 0; j < count; j++) {
+    jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], (n[j] + fapk::BT - 1) / fapk::BT};
+    ops += 2 * (int64_t)n[j] * n[j] * n[j];
+  }
+  fap_fw_plan *p = new fap_fw_plan();
+  p->ctx = ctx;
+  p->njobs = count;
+  p->group_off = goff;
+  p->ops = ops;
+  const size_t jobs_b = sizeof(fapk::BatchJob) * count;
+  p->tasks_off = (jobs_b + 255) & ~(size_t)255;
+  const size_t total = p->tasks_off + sizeof(int4) * std::max<size_t>(tasks.size(), 1);
+  std::vector<char> host(total, 0);
+  if (jobs_b) memcpy(host.data(), jobs.data(), jobs_b);
+  if (!tasks.empty()) memcpy(host.data() + p->tasks_off, tasks.data(), sizeof(int4) * tasks.size());
+  hipError_t e = hipMalloc(&p->desc, total);
+  if (e == hipSuccess) e = hipMemcpy(p->desc, host.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (p->desc) (void)hipFree(p->desc);
+    delete p;
+    return hip_fail(e, "batch plan descriptors", __FILE__, __LINE__);
+  }
+  if (use_graph && !tasks.empty()) {
+    hipStream_t cs = ctx->aux_stream;  // capture on a stream with no other work queued
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = FAP_OK;
+    if (e == hipSuccess) rc = plan_launch(p, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess && rc == FAP_OK) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess || rc != FAP_OK) {
+      if (p->graph) (void)hipGraphDestroy(p->graph);
+      (void)hipFree(p->desc);
+      delete p;
+      return rc != FAP_OK ? rc : hip_fail(e, "batch plan graph capture", __FILE__, __LINE__);
+    }
+  }
+  *out = p;
+  return FAP_OK;
+}
+
+int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
+  if (p->exec) {
+    FAP_HIP(hipGraphLaunch(p->exec, s));
+    return FAP_OK;
+  }
+  return plan_launch(p, s);
+}
+
+void fw_plan_destroy(fap_fw_plan *p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->desc) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->desc);
+  }
+  delete p;
+}
+
+static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, const int64_t *ld) {
+  if (count < 0 || (count > 0 && (!d_mats || !n || !ld)))
+    return fail(FAP_E_ARG, "fw_batched: bad arguments");
+  int64_t tiles = 0;
+  for (int32_t j = 0; j < count; j++) {
+    if (n[j] < 0 || (n[j] > 0 && (!d_mats[j] || ld[j] < n[j])))
+      return fail(FAP_E_ARG, "fw_batched: job " + std::to_string(j) + " has a bad size/stride");
+    const int64_t nb = (n[j] + 63) / 64;
+    tiles += nb * nb * nb;
+  }
+  if (tiles > ((int64_t)1 << 28)) return fail(FAP_E_ARG, "fw_batched: batch too large");
+  return FAP_OK;
+}
+
+}  // namespace fapi
+
+using namespace fapi;
+
+extern "C" {
+
+int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                   const int64_t *ld, void *stream) {
+  if (!ctx) return fail(FAP_E_ARG, "ctx is null");
+  FAP_TRY(check_jobs(count, d_mats, n, ld));
+  if (count == 0) return FAP_OK;
+  fap_fw_plan *p = nullptr;
+  FAP_TRY(fw_plan_create(ctx, count, d_mats, n, ld, 0, &p));
+  hipStream_t s = pick_stream(ctx, stream);
+  int rc = fw_plan_run(p, s);
+  if (rc == FAP_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "fw_batched", __FILE__, __LINE__);
+  }
+  fw_plan_destroy(p);
+  return rc;
+}
+
+int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                       const int64_t *ld, int use_graph, fap_fw_plan **out) {
+  if (!ctx || !out) return fail(FAP_E_ARG, "null argument");
+  FAP_TRY(check_jobs(count, d_mats, n, ld));
+  return fw_plan_create(ctx, count, d_mats, n, ld, use_graph, out);
+}
+
+int fap_fw_plan_run(fap_fw_plan *plan, void *stream) {
+  if (!plan) return fail(FAP_E_ARG, "null plan");
+  return fw_plan_run(plan, pick_stream(plan->ctx, stream));
+}
+
+int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks, int64_t *ops) {
+  if (!plan) return fail(FAP_E_ARG, "null plan");
+  if (launches) *launches = (int32_t)plan->group_off.size() - 1;
+  if (tasks) *tasks = plan->group_off.empty() ? 0 : plan->group_off.back();
+  if (ops) *ops = plan->ops;
+  return FAP_OK;
+}
+
+int fap_fw_plan_destroy(fap_fw_plan *plan) {
+  fw_plan_destroy(plan);
+  return FAP_OK;
+}
+
+}  // extern "C"

@@ -57,6 +57,8 @@ struct fap_graph_dev {
   int32_t *st2ed = nullptr;       // V
   int32_t *ed2st = nullptr;       // V
   int32_t max_degree = 0;
+  int64_t weight_sum = 0;
+  int32_t delta = 0;  // SSSP near-far band width (0 = derived from the mean weight)
 };
 
 struct fap_ctx {
@@ -71,8 +73,19 @@ struct fap_ctx {
   fapi::Workspace ws_stamp;   // SSSP enqueue stamps
   fapi::Workspace ws_small;   // counters etc.
   fapi::Workspace ws_misc;
+  fapi::Workspace ws_batch;   // batched-closure descriptors and counters
+  fapi::Workspace ws_dense;   // dense-build descriptors
   uint32_t *host_pinned = nullptr;  // small pinned host buffer for counters
   hipEvent_t ev_poll = nullptr;
+  // second stream and events for the Floyd-Warshall look-ahead
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fw_la = nullptr;
+  hipEvent_t ev_fw_panel = nullptr;
+  // tuning options (fap_ctx_set_option); they change speed, never results
+  int opt_fw_lookahead = 0;      // run round r+1's phases 1-2 beside round r's phase 3
+  int opt_fw_tile = 0;           // FW block size 64 / 128; 0 = by n
+  int opt_sssp_threads = 1024;   // threads per SSSP workgroup: 256, 512, 1024
+  int opt_sssp_delta = 0;        // near-far band; 0 = per graph (4 x mean weight)
 };
 
 namespace fapi {

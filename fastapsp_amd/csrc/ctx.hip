@@ -153,6 +153,9 @@ int fap_ctx_create(int device, fap_ctx **out) {
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->host_pinned, 4096, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_poll, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_la, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_panel, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "context setup", __FILE__, __LINE__);
@@ -166,13 +169,49 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   Workspace *all[] = {&ctx->ws_sub, &ctx->ws_queue, &ctx->ws_stamp, &ctx->ws_small,
-                      &ctx->ws_misc};
+                      &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense};
   for (Workspace *w : all)
     if (w->ptr) (void)hipFree(w->ptr);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->ev_poll) (void)hipEventDestroy(ctx->ev_poll);
+  if (ctx->ev_fw_la) (void)hipEventDestroy(ctx->ev_fw_la);
+  if (ctx->ev_fw_panel) (void)hipEventDestroy(ctx->ev_fw_panel);
+  if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return FAP_OK;
+}
+
+int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
+  if (!ctx || !name) return fail(FAP_E_ARG, "null argument");
+  const std::string k(name);
+  if (k == "fw_lookahead") {
+    ctx->opt_fw_lookahead = value != 0;
+  } else if (k == "fw_tile") {
+    if (value != 0 && value != 64 && value != 128)
+      return fail(FAP_E_ARG, "fw_tile must be 0, 64 or 128");
+    ctx->opt_fw_tile = (int)value;
+  } else if (k == "sssp_threads") {
+    if (value != 256 && value != 512 && value != 1024)
+      return fail(FAP_E_ARG, "sssp_threads must be 256, 512 or 1024");
+    ctx->opt_sssp_threads = (int)value;
+  } else if (k == "sssp_delta") {
+    if (value < 0 || value > INT32_MAX / 4) return fail(FAP_E_ARG, "sssp_delta out of range");
+    ctx->opt_sssp_delta = (int)value;
+  } else {
+    return fail(FAP_E_ARG, "unknown option '" + k + "'");
+  }
+  return FAP_OK;
+}
+
+int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
+  if (!ctx || !name || !value) return fail(FAP_E_ARG, "null argument");
+  const std::string k(name);
+  if (k == "fw_lookahead") *value = ctx->opt_fw_lookahead;
+  else if (k == "fw_tile") *value = ctx->opt_fw_tile;
+  else if (k == "sssp_threads") *value = ctx->opt_sssp_threads;
+  else if (k == "sssp_delta") *value = ctx->opt_sssp_delta;
+  else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }
 
@@ -330,18 +369,21 @@ int fap_graph_upload(fap_ctx *ctx, int32_t V, int64_t E, const int32_t *row_offs
     maxdeg = std::max(maxdeg, d);
   }
   std::vector<int2> packed((size_t)E);
+  int64_t wsum = 0;
   for (int64_t e = 0; e < E; e++) {
     if (col[e] < 0 || col[e] >= V)
       return fail(FAP_E_ARG, "col[" + std::to_string(e) + "] out of range");
     if (w[e] < 0 || w[e] >= FAP_INF)
       return fail(FAP_E_ARG, "weight[" + std::to_string(e) + "] outside [0, INF)");
     packed[e] = make_int2(col[e], w[e]);
+    wsum += w[e];
   }
   FAP_HIP(hipSetDevice(ctx->device));
   fap_graph_dev *g = new fap_graph_dev();
   g->V = V;
   g->E = E;
   g->max_degree = maxdeg;
+  g->weight_sum = wsum;
   hipError_t e1 = hipMalloc(&g->row_offset, sizeof(int32_t) * (V + 1));
   hipError_t e2 = hipMalloc(&g->edges, sizeof(int2) * (E > 0 ? E : 1));
   if (e1 != hipSuccess || e2 != hipSuccess) {

@@ -3,135 +3,179 @@
 // Replaces the reference's stage-2 operator floyd_path (fap/kernel/floyd.h:
 // 150-158; CPU semantics floyd_CPU_path :38-49: k-i-j order, strict relax).
 // The closure of a non-negative matrix is unique, so the blocked order gives
-// the same distances as the k-i-j loop.  With block size T = 128 and
+// the same distances as the k-i-j loop.  With block size T (64 or 128) and
 // nb = ceil(n / T) blocks, round r (pivot block r) runs:
-//   1. diagonal tile (r, r): FW over its kb <= T pivots in one workgroup,
-//      cells in registers, pivot row/column exchanged through LDS;
+//   1. diagonal tile (r, r): closed in one workgroup, in LDS;
 //   2. row panel (r, J) = min(C, D_rr (x) C) and column panel
 //      (I, r) = min(C, C (x) D_rr) for all I, J != r: one launch;
 //   3. every other tile (I, J) = min(C, D_Ir (x) D_rJ): one launch.
-// Phases 2 and 3 reuse the register-blocked min-plus tile (minplus_tile.h).
+// Phases 2 and 3 reuse the register-blocked min-plus tile (minplus_tile.h)
+// with T x T tiles.  T = 128 halves the number of rounds (less HBM traffic
+// per closure); T = 64 gives 4x more tiles per launch and a cheap diagonal
+// phase, and is the default (faster at every measured n).
 #include "common.h"
 #include "minplus_tile.h"
 
 namespace fapk {
 
-constexpr int FW_T = 128;
 constexpr int FW_DIAG_THREADS = 1024;
 
-// Phase 1.  Thread (ty, tx) = (t >> 5, t & 31) owns cells (ty + 32a, tx + 32b),
-// a, b < 4.  Before step k the owners of row k / column k publish them into
-// rowbuf / colbuf (double-buffered by k parity), one barrier per step.  Row k
-// and column k do not change during step k because the diagonal is >= 0.
-__global__ __launch_bounds__(FW_DIAG_THREADS) void fw_diag_kernel(int32_t *d, int n,
-                                                                  int64_t ld, int r) {
-  __shared__ int32_t rowbuf[2][FW_T];
-  __shared__ int32_t colbuf[2][FW_T];
+// In-LDS helpers for the diagonal tile (1024 threads).  The tile lives in
+// D[T][T + 4] (rows 16-byte aligned).  A thread owns a 1 x 4 strip of a
+// 64 x 64 sub-block: row t >> 4, columns (t & 15) * 4 .. +3.
+
+// Closure of the 64 x 64 sub-block at (o, o), pivots o .. o + nv - 1.  Row k
+// and column k do not change during step k (the diagonal is >= 0), so every
+// thread can update its strip in place; one barrier per pivot.
+template <int LD>
+__device__ __forceinline__ void lds_fw64(int32_t *D, int o, int nv) {
   const int t = threadIdx.x;
-  const int ty = t >> 5, tx = t & 31;
-  const int base = r * FW_T;
-  const int kb = imin(FW_T, n - base);
-  int32_t *blk = d + (int64_t)base * ld + base;
-  int cell[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int i = ty + 32 * a, j = tx + 32 * b;
-      cell[a][b] = (i < kb && j < kb) ? blk[(int64_t)i * ld + j] : kInf;
-    }
-  // publish pivot 0 (row 0 is owned by ty == 0, a == 0; column 0 by tx == 0, b == 0)
-  if (ty == 0) {
-#pragma unroll
-    for (int b = 0; b < 4; b++) rowbuf[0][tx + 32 * b] = cell[0][b];
-  }
-  if (tx == 0) {
-#pragma unroll
-    for (int a = 0; a < 4; a++) colbuf[0][ty + 32 * a] = cell[a][0];
-  }
-  __syncthreads();
-  for (int k = 0; k < kb; k++) {
-    const int p = k & 1;
-    int pr[4], pc[4];
-#pragma unroll
-    for (int b = 0; b < 4; b++) pr[b] = rowbuf[p][tx + 32 * b];
-#pragma unroll
-    for (int a = 0; a < 4; a++) pc[a] = colbuf[p][ty + 32 * a];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) cell[a][b] = imin(cell[a][b], pc[a] + pr[b]);
-    const int nx = k + 1;
-    if (nx < kb) {
-      const int q = nx & 1;
-      if ((nx & 31) == ty) {
-        const int asel = nx >> 5;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-          if (a == asel) {
-#pragma unroll
-            for (int b = 0; b < 4; b++) rowbuf[q][tx + 32 * b] = cell[a][b];
-          }
-      }
-      if ((nx & 31) == tx) {
-        const int bsel = nx >> 5;
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-          if (b == bsel) {
-#pragma unroll
-            for (int a = 0; a < 4; a++) colbuf[q][ty + 32 * a] = cell[a][b];
-          }
-      }
-    }
+  const int i = o + (t >> 4), j = o + ((t & 15) << 2);
+  int4 *cp = reinterpret_cast<int4 *>(D + i * LD + j);
+  for (int k = o; k < o + nv; k++) {
+    const int a = D[i * LD + k];
+    const int4 b = *reinterpret_cast<const int4 *>(D + k * LD + j);
+    int4 c = *cp;
+    c.x = imin(c.x, a + b.x);
+    c.y = imin(c.y, a + b.y);
+    c.z = imin(c.z, a + b.z);
+    c.w = imin(c.w, a + b.w);
+    *cp = c;
     __syncthreads();
   }
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int i = ty + 32 * a, j = tx + 32 * b;
-      if (i < kb && j < kb) blk[(int64_t)i * ld + j] = cell[a][b];
-    }
+}
+
+// C = min(C, A (x) B) for 64 x 64 sub-blocks given by their corners
+// (row, col) in D, K = 64.  All reads happen before the barrier that
+// precedes the writes, so C may alias A or B.
+template <int LD>
+__device__ __forceinline__ void lds_mp64(int32_t *D, int ci, int cj, int ai, int ak, int bk,
+                                         int bj) {
+  const int t = threadIdx.x;
+  const int r = t >> 4, c0 = (t & 15) << 2;
+  int4 acc = *reinterpret_cast<const int4 *>(D + (ci + r) * LD + cj + c0);
+  const int32_t *arow = D + (ai + r) * LD + ak;
+#pragma unroll 4
+  for (int k = 0; k < 64; k += 2) {
+    const int2 a = *reinterpret_cast<const int2 *>(arow + k);
+    const int4 b0 = *reinterpret_cast<const int4 *>(D + (bk + k) * LD + bj + c0);
+    const int4 b1 = *reinterpret_cast<const int4 *>(D + (bk + k + 1) * LD + bj + c0);
+    acc.x = imin(acc.x, imin(a.x + b0.x, a.y + b1.x));
+    acc.y = imin(acc.y, imin(a.x + b0.y, a.y + b1.y));
+    acc.z = imin(acc.z, imin(a.x + b0.z, a.y + b1.z));
+    acc.w = imin(acc.w, imin(a.x + b0.w, a.y + b1.w));
+  }
+  __syncthreads();
+  *reinterpret_cast<int4 *>(D + (ci + r) * LD + cj + c0) = acc;
+  __syncthreads();
+}
+
+// Phase 1 for T = 64 (direct) or T = 128 (2 x 2 recursion on 64-blocks:
+// close D00; panels D01, D10; D11 += D10 D01; close D11; panels D10, D01;
+// D00 += D01 D10 -- the blocked algorithm itself, one level down, so most
+// of the work runs as min-plus products with two k per v_min3).
+template <int T>
+__global__ __launch_bounds__(FW_DIAG_THREADS) void fw_diag_kernel(int32_t *d, int n,
+                                                                  int64_t ld, int r) {
+  constexpr int LD = T + 4;
+  __shared__ __attribute__((aligned(16))) int32_t D[T * LD];
+  const int base = r * T;
+  const int kb = imin(T, n - base);
+  int32_t *blk = d + (int64_t)base * ld + base;
+  for (int idx = threadIdx.x; idx < T * T; idx += FW_DIAG_THREADS) {
+    const int i = idx / T, j = idx - (idx / T) * T;
+    D[i * LD + j] = (i < kb && j < kb) ? blk[(int64_t)i * ld + j] : kInf;
+  }
+  __syncthreads();
+  const int n0 = imin(64, kb);
+  lds_fw64<LD>(D, 0, n0);
+  if (T == 128 && kb > 64) {
+    const int n1 = kb - 64;
+    lds_mp64<LD>(D, 0, 64, 0, 0, 0, 64);    // D01 = min(D01, D00 D01)
+    lds_mp64<LD>(D, 64, 0, 64, 0, 0, 0);    // D10 = min(D10, D10 D00)
+    lds_mp64<LD>(D, 64, 64, 64, 0, 0, 64);  // D11 = min(D11, D10 D01)
+    lds_fw64<LD>(D, 64, n1);
+    lds_mp64<LD>(D, 64, 0, 64, 64, 64, 0);  // D10 = min(D10, D11 D10)
+    lds_mp64<LD>(D, 0, 64, 0, 64, 64, 64);  // D01 = min(D01, D01 D11)
+    lds_mp64<LD>(D, 0, 0, 0, 64, 64, 0);    // D00 = min(D00, D01 D10)
+  }
+  for (int idx = threadIdx.x; idx < T * T; idx += FW_DIAG_THREADS) {
+    const int i = idx / T, j = idx - (idx / T) * T;
+    if (i < kb && j < kb) blk[(int64_t)i * ld + j] = D[i * LD + j];
+  }
 }
 
 // Phase 2: workgroups [0, nb-1) take row-panel tiles (r, J), the next nb-1
 // take column-panel tiles (I, r).  Each tile is owned by one workgroup and
-// aliases one operand; minplus_tile reads all operands before writing C.
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void fw_panel_kernel(int32_t *d, int n, int64_t ld,
-                                                              int r, int nb, int flags) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MP_LDS_WORDS];
-  const int base = r * FW_T;
-  const int kb = imin(FW_T, n - base);
+// aliases one operand; the tile reads all operands before writing C.
+template <int TM>
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_panel_kernel(
+    int32_t *d, int n, int64_t ld, int r, int nb, int flags) {
+  using Tile = MpTile<TM, TM>;
+  constexpr int T = Tile::BM;
+  __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
+  const int base = r * T;
+  const int kb = imin(T, n - base);
   const int32_t *diag = d + (int64_t)base * ld + base;
   int bid = blockIdx.x;
   if (bid < nb - 1) {
     const int J = bid < r ? bid : bid + 1;
-    int32_t *c = d + (int64_t)base * ld + (int64_t)J * FW_T;
-    minplus_tile(diag, ld, c, ld, c, ld, kb, imin(FW_T, n - J * FW_T), kb, true, flags, lds);
+    int32_t *c = d + (int64_t)base * ld + (int64_t)J * T;
+    Tile::run(diag, ld, c, ld, c, ld, kb, imin(T, n - J * T), kb, true, flags, lds);
   } else {
     bid -= nb - 1;
     const int I = bid < r ? bid : bid + 1;
-    int32_t *c = d + (int64_t)I * FW_T * ld + base;
-    minplus_tile(c, ld, diag, ld, c, ld, imin(FW_T, n - I * FW_T), kb, kb, true, flags, lds);
+    int32_t *c = d + (int64_t)I * T * ld + base;
+    Tile::run(c, ld, diag, ld, c, ld, imin(T, n - I * T), kb, kb, true, flags, lds);
   }
 }
 
-// Phase 3: the (nb-1)^2 tiles outside row r and column r.
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void fw_rest_kernel(int32_t *d, int n, int64_t ld,
-                                                             int r, int nb, int flags) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MP_LDS_WORDS];
-  const int m = nb - 1;
-  const int t = xcd_remap(blockIdx.x, m * m);
-  int I = t / m, J = t - (t / m) * m;
-  I += (I >= r);
-  J += (J >= r);
-  const int base = r * FW_T;
-  const int kb = imin(FW_T, n - base);
-  int32_t *c = d + (int64_t)I * FW_T * ld + (int64_t)J * FW_T;
-  const int32_t *a = d + (int64_t)I * FW_T * ld + base;
-  const int32_t *b = d + (int64_t)base * ld + (int64_t)J * FW_T;
-  minplus_tile(a, ld, b, ld, c, ld, imin(FW_T, n - I * FW_T), imin(FW_T, n - J * FW_T), kb,
-               true, flags, lds);
+// Phase 3 over a set of tiles, selected by `mode` (p = r + 1):
+//   0: every tile outside row r and column r, (nb-1)^2 of them;
+//   1: the look-ahead tiles, row p and column p minus row/column r (2nb-3);
+//   2: the rest, outside rows/columns r and p, (nb-2)^2.
+// Modes 1 and 2 together equal mode 0; splitting them lets the next round's
+// diagonal and panels (which only touch row/column p) start while mode 2
+// runs.
+template <int TM>
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_rest_kernel(
+    int32_t *d, int n, int64_t ld, int r, int nb, int flags, int mode) {
+  using Tile = MpTile<TM, TM>;
+  constexpr int T = Tile::BM;
+  __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
+  int I, J;
+  if (mode == 0) {
+    const int m = nb - 1;
+    const int t = xcd_remap(blockIdx.x, m * m);
+    I = t / m;
+    J = t - I * m;
+    I += (I >= r);
+    J += (J >= r);
+  } else if (mode == 1) {
+    const int p = r + 1;
+    const int t = blockIdx.x;
+    if (t < nb - 1) {
+      I = p;
+      J = t + (t >= r);
+    } else {
+      const int u = t - (nb - 1);
+      I = u < r ? u : u + 2;
+      J = p;
+    }
+  } else {
+    const int m = nb - 2;
+    const int t = xcd_remap(blockIdx.x, m * m);
+    I = t / m;
+    J = t - I * m;
+    I = I < r ? I : I + 2;
+    J = J < r ? J : J + 2;
+  }
+  const int base = r * T;
+  const int kb = imin(T, n - base);
+  int32_t *c = d + (int64_t)I * T * ld + (int64_t)J * T;
+  const int32_t *a = d + (int64_t)I * T * ld + base;
+  const int32_t *b = d + (int64_t)base * ld + (int64_t)J * T;
+  Tile::run(a, ld, b, ld, c, ld, imin(T, n - I * T), imin(T, n - J * T), kb, true, flags, lds);
 }
 
 }  // namespace fapk
@@ -141,24 +185,79 @@ namespace fapi {
 int vec_flags(const void *A, int64_t lda, const void *B, int64_t ldb, const void *C,
               int64_t ldc);
 
-int launch_fw(int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
-  if (n <= 0) return FAP_OK;
-  const int nb = (n + fapk::FW_T - 1) / fapk::FW_T;
-  const int flags = vec_flags(d, ld, d, ld, d, ld);
-  for (int r = 0; r < nb; r++) {
-    hipLaunchKernelGGL(fapk::fw_diag_kernel, dim3(1), dim3(fapk::FW_DIAG_THREADS), 0, s, d, n,
-                       ld, r);
+namespace {
+
+template <int TM>
+struct Fw {
+  static constexpr int T = 16 * TM;
+
+  static int diag_panel(int32_t *d, int32_t n, int64_t ld, int r, int nb, int flags,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(fapk::fw_diag_kernel<T>, dim3(1), dim3(fapk::FW_DIAG_THREADS), 0, s, d,
+                       n, ld, r);
     FAP_CHECK_LAUNCH();
     if (nb > 1) {
-      hipLaunchKernelGGL(fapk::fw_panel_kernel, dim3(2 * (nb - 1)), dim3(fapk::MP_THREADS), 0,
-                         s, d, n, ld, r, nb, flags);
-      FAP_CHECK_LAUNCH();
-      hipLaunchKernelGGL(fapk::fw_rest_kernel, dim3((nb - 1) * (nb - 1)),
-                         dim3(fapk::MP_THREADS), 0, s, d, n, ld, r, nb, flags);
+      hipLaunchKernelGGL(fapk::fw_panel_kernel<TM>, dim3(2 * (nb - 1)), dim3(fapk::MP_THREADS),
+                         0, s, d, n, ld, r, nb, flags);
       FAP_CHECK_LAUNCH();
     }
+    return FAP_OK;
   }
-  return FAP_OK;
+
+  static int rest(int32_t *d, int32_t n, int64_t ld, int r, int nb, int flags, int mode,
+                  hipStream_t s) {
+    const int tiles =
+        mode == 0 ? (nb - 1) * (nb - 1) : mode == 1 ? 2 * nb - 3 : (nb - 2) * (nb - 2);
+    if (tiles <= 0) return FAP_OK;
+    hipLaunchKernelGGL(fapk::fw_rest_kernel<TM>, dim3(tiles), dim3(fapk::MP_THREADS), 0, s, d,
+                       n, ld, r, nb, flags, mode);
+    FAP_CHECK_LAUNCH();
+    return FAP_OK;
+  }
+
+  // With look-ahead, round r's phase 3 is split: the tiles of row/column
+  // r+1 first (mode 1), then, on the auxiliary stream, round r+1's diagonal
+  // and panels run while the main stream finishes round r's other tiles
+  // (mode 2).  All work is ordered before later work on `s` on return.
+  static int run(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
+    const int nb = (n + T - 1) / T;
+    const int flags = vec_flags(d, ld, d, ld, d, ld);
+    FAP_TRY(diag_panel(d, n, ld, 0, nb, flags, s));
+    const bool overlap = ctx && ctx->opt_fw_lookahead && ctx->aux_stream && nb >= 3;
+    for (int r = 0; r < nb; r++) {
+      if (r + 1 < nb && overlap) {
+        FAP_TRY(rest(d, n, ld, r, nb, flags, 1, s));
+        FAP_HIP(hipEventRecord(ctx->ev_fw_la, s));
+        FAP_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fw_la, 0));
+        FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, ctx->aux_stream));
+        FAP_HIP(hipEventRecord(ctx->ev_fw_panel, ctx->aux_stream));
+        FAP_TRY(rest(d, n, ld, r, nb, flags, 2, s));
+        FAP_HIP(hipStreamWaitEvent(s, ctx->ev_fw_panel, 0));
+      } else {
+        FAP_TRY(rest(d, n, ld, r, nb, flags, 0, s));
+        if (r + 1 < nb) FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, s));
+      }
+    }
+    return FAP_OK;
+  }
+};
+
+}  // namespace
+
+// Block size: the context option if set, else 64.  Measured on MI355X
+// (bench.py --workload fwsweep), 64 beats 128 from n = 256 to n = 8192: 4x
+// more tiles per launch and a diagonal phase that is 8x cheaper outweigh
+// twice the rounds.
+int fw_block_size(const fap_ctx *ctx, int32_t n) {
+  (void)n;
+  if (ctx && (ctx->opt_fw_tile == 64 || ctx->opt_fw_tile == 128)) return ctx->opt_fw_tile;
+  return 64;
+}
+
+int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
+  if (n <= 0) return FAP_OK;
+  if (fw_block_size(ctx, n) == 128) return Fw<8>::run(ctx, d, n, ld, s);
+  return Fw<4>::run(ctx, d, n, ld, s);
 }
 
 }  // namespace fapi
@@ -172,7 +271,6 @@ extern "C" int fap_fw_blocked(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t l
   if (n == 0) return FAP_OK;
   if (!d_mat) return fail(FAP_E_ARG, "fw: null matrix");
   if (ld < n) return fail(FAP_E_ARG, "fw: ld < n");
-  if ((int64_t)(n + fapk::FW_T - 1) / fapk::FW_T > 46340)
-    return fail(FAP_E_ARG, "fw: n too large");
-  return launch_fw(d_mat, n, ld, pick_stream(ctx, stream));
+  if ((int64_t)(n + 63) / 64 > 46340) return fail(FAP_E_ARG, "fw: n too large");
+  return launch_fw(ctx, d_mat, n, ld, pick_stream(ctx, stream));
 }

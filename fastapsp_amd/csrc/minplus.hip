@@ -28,12 +28,12 @@ __device__ __forceinline__ void gemm_tile_coords(int bid, int tiles_m, int tiles
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void minplus_gemm_kernel(
     const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C, int64_t ldc,
     int M, int N, int K, int accumulate, int flags, int tiles_m, int tiles_n) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MP_LDS_WORDS];
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpBig::LDS_WORDS];
   int tm, tn;
   gemm_tile_coords(blockIdx.x, tiles_m, tiles_n, &tm, &tn);
   const int m0 = tm * MP_BM, n0 = tn * MP_BN;
-  minplus_tile(A + (int64_t)m0 * lda, lda, B + n0, ldb, C + (int64_t)m0 * ldc + n0, ldc,
-               imin(MP_BM, M - m0), imin(MP_BN, N - n0), K, accumulate != 0, flags, lds);
+  MpBig::run(A + (int64_t)m0 * lda, lda, B + n0, ldb, C + (int64_t)m0 * ldc + n0, ldc,
+             imin(MP_BM, M - m0), imin(MP_BN, N - n0), K, accumulate != 0, flags, lds);
 }
 
 }  // namespace fapk

@@ -1,20 +1,20 @@
 // minplus_tile.h -- register-blocked VALU min-plus tile for gfx950 (wave64).
 //
-// One workgroup of 256 threads (4 waves) computes a 128 x 128 tile
+// One workgroup of 256 threads (4 waves) computes a 128 x 128 (or 64 x 64) tile
 //   C[i][j] = min(base, min_k A[i][k] + B[k][j]),  base = C[i][j] or INF.
 // Min-plus is not a multiply-accumulate, so it runs on the VALU, not MFMA:
 // every update is one v_add_u32, and two consecutive k are folded with one
 // v_min3_i32, i.e. 1.5 VALU instructions per update.
 //
 // Layout:
-//   * waves form a 2 x 2 grid of 64 x 64 sub-tiles; a lane owns an 8 x 8
-//     micro-tile: rows ly + 8r (r < 8, interleaved so that the 8 distinct rows
-//     a wave reads from LDS fall into distinct banks) and columns
-//     lx*4 + 32h + c (h < 2, c < 4: two 16-byte groups, so each group of 8
-//     lanes reads 128 contiguous bytes of a B row).
+//   * waves form a 2 x 2 grid of sub-tiles; a lane owns a TM x TN
+//     micro-tile (8 x 8 or 4 x 4): rows ly + 8r (r < TM, interleaved so that
+//     the 8 distinct rows a wave reads from LDS fall into distinct banks) and
+//     columns lx*4 + 32h + c (h < TN/4, c < 4: 16-byte groups, so each group
+//     of 8 lanes reads 128 contiguous bytes of a B row).
 //   * k is staged through LDS in chunks of 16: A as As[m][k] (row stride 20
-//     words), B as Bs[k][n] (row stride 132 words); global loads for chunk
-//     c+1 are issued before chunk c is computed.
+//     words), B as Bs[k][n] (row stride BN + 4 words); global loads for
+//     chunk c+1 are issued before chunk c is computed.
 //   * out-of-range rows, columns and k read as INF, so ragged M, N, K need no
 //     padding (INF + anything >= INF never wins a strict min).
 #ifndef FAP_MINPLUS_TILE_H_
@@ -26,13 +26,9 @@
 namespace fapk {
 
 constexpr int kInf = 0x3FFFFFFF;
-constexpr int MP_BM = 128;
-constexpr int MP_BN = 128;
 constexpr int MP_BK = 16;
 constexpr int MP_THREADS = 256;
-constexpr int MP_AS_LD = MP_BK + 4;   // words per A row in LDS
-constexpr int MP_BS_LD = MP_BN + 4;   // words per B row in LDS
-constexpr int MP_LDS_WORDS = MP_BM * MP_AS_LD + MP_BK * MP_BS_LD;
+constexpr int MP_AS_LD = MP_BK + 4;  // words per A row in LDS
 
 // Flags describing what 16-byte vector accesses are legal for a launch.
 enum : int {
@@ -43,151 +39,172 @@ enum : int {
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
-// Stage one 16-wide k chunk of A (tile rows) and B (tile columns) into
-// registers.  Thread t: A row t>>1, k (t&1)*8..+8; B k-row t>>4, cols (t&15)*8..+8.
-__device__ __forceinline__ void mp_load_chunk(const int32_t *__restrict__ A, int64_t lda,
-                                              const int32_t *__restrict__ B, int64_t ldb,
-                                              int m_valid, int n_valid, int K, int k0,
-                                              int flags, int ra[8], int rb[8]) {
-  const int t = threadIdx.x;
-  {
-    const int row = t >> 1;
-    const int kq = k0 + ((t & 1) << 3);
-    const int32_t *p = A + (int64_t)row * lda + kq;
-    if (row < m_valid && kq + 8 <= K && (flags & MP_VEC_A)) {
-      int4 v0 = *reinterpret_cast<const int4 *>(p);
-      int4 v1 = *reinterpret_cast<const int4 *>(p + 4);
-      ra[0] = v0.x; ra[1] = v0.y; ra[2] = v0.z; ra[3] = v0.w;
-      ra[4] = v1.x; ra[5] = v1.y; ra[6] = v1.z; ra[7] = v1.w;
-    } else {
+// Tile geometry: each lane owns TM x TN outputs (TM, TN in {4, 8}); the 4
+// waves form a 2 x 2 grid, so the workgroup tile is (16 TM) x (16 TN):
+// 128 x 128 for 8 x 8, 64 x 64 for 4 x 4.
+template <int TM, int TN>
+struct MpTile {
+  static constexpr int BM = 16 * TM;
+  static constexpr int BN = 16 * TN;
+  static constexpr int BS_LD = BN + 4;  // words per B row in LDS
+  static constexpr int LDS_WORDS = BM * MP_AS_LD + MP_BK * BS_LD;
+  static constexpr int A_TPR = MP_BK / TM;  // threads per A row when staging
+
+  // Stage one 16-wide k chunk into registers: A row t / A_TPR, k
+  // (t % A_TPR) * TM .. +TM; B k-row t / 16, columns (t % 16) * TN .. +TN.
+  __device__ static __forceinline__ void load_chunk(const int32_t *A, int64_t lda,
+                                                    const int32_t *B, int64_t ldb, int m_valid,
+                                                    int n_valid, int K, int k0, int flags,
+                                                    int ra[TM], int rb[TN]) {
+    const int t = threadIdx.x;
+    {
+      const int row = t / A_TPR;
+      const int kq = k0 + (t % A_TPR) * TM;
+      const int32_t *p = A + (int64_t)row * lda + kq;
+      if (row < m_valid && kq + TM <= K && (flags & MP_VEC_A)) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) ra[i] = (row < m_valid && kq + i < K) ? p[i] : kInf;
+        for (int q = 0; q < TM; q += 4) {
+          const int4 v = *reinterpret_cast<const int4 *>(p + q);
+          ra[q] = v.x; ra[q + 1] = v.y; ra[q + 2] = v.z; ra[q + 3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; i++) ra[i] = (row < m_valid && kq + i < K) ? p[i] : kInf;
+      }
+    }
+    {
+      const int kr = k0 + (t >> 4);
+      const int col = (t & 15) * TN;
+      const int32_t *p = B + (int64_t)kr * ldb + col;
+      if (kr < K && col + TN <= n_valid && (flags & MP_VEC_B)) {
+#pragma unroll
+        for (int q = 0; q < TN; q += 4) {
+          const int4 v = *reinterpret_cast<const int4 *>(p + q);
+          rb[q] = v.x; rb[q + 1] = v.y; rb[q + 2] = v.z; rb[q + 3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TN; i++) rb[i] = (kr < K && col + i < n_valid) ? p[i] : kInf;
+      }
     }
   }
-  {
-    const int kr = k0 + (t >> 4);
-    const int col = (t & 15) << 3;
-    const int32_t *p = B + (int64_t)kr * ldb + col;
-    if (kr < K && col + 8 <= n_valid && (flags & MP_VEC_B)) {
-      int4 v0 = *reinterpret_cast<const int4 *>(p);
-      int4 v1 = *reinterpret_cast<const int4 *>(p + 4);
-      rb[0] = v0.x; rb[1] = v0.y; rb[2] = v0.z; rb[3] = v0.w;
-      rb[4] = v1.x; rb[5] = v1.y; rb[6] = v1.z; rb[7] = v1.w;
-    } else {
+
+  __device__ static __forceinline__ void store_chunk(int32_t *As, int32_t *Bs, const int ra[TM],
+                                                     const int rb[TN]) {
+    const int t = threadIdx.x;
+    int32_t *pa = As + (t / A_TPR) * MP_AS_LD + (t % A_TPR) * TM;
 #pragma unroll
-      for (int i = 0; i < 8; i++) rb[i] = (kr < K && col + i < n_valid) ? p[i] : kInf;
-    }
+    for (int q = 0; q < TM; q += 4)
+      *reinterpret_cast<int4 *>(pa + q) = make_int4(ra[q], ra[q + 1], ra[q + 2], ra[q + 3]);
+    int32_t *pb = Bs + (t >> 4) * BS_LD + (t & 15) * TN;
+#pragma unroll
+    for (int q = 0; q < TN; q += 4)
+      *reinterpret_cast<int4 *>(pb + q) = make_int4(rb[q], rb[q + 1], rb[q + 2], rb[q + 3]);
   }
-}
 
-__device__ __forceinline__ void mp_store_chunk(int32_t *As, int32_t *Bs, const int ra[8],
-                                               const int rb[8]) {
-  const int t = threadIdx.x;
-  int32_t *pa = As + (t >> 1) * MP_AS_LD + ((t & 1) << 3);
-  *reinterpret_cast<int4 *>(pa) = make_int4(ra[0], ra[1], ra[2], ra[3]);
-  *reinterpret_cast<int4 *>(pa + 4) = make_int4(ra[4], ra[5], ra[6], ra[7]);
-  int32_t *pb = Bs + (t >> 4) * MP_BS_LD + ((t & 15) << 3);
-  *reinterpret_cast<int4 *>(pb) = make_int4(rb[0], rb[1], rb[2], rb[3]);
-  *reinterpret_cast<int4 *>(pb + 4) = make_int4(rb[4], rb[5], rb[6], rb[7]);
-}
-
-// Compute the tile.  A points at the tile's first row (k = 0), B at the
-// tile's first column (k = 0), C at the tile's corner.  m_valid / n_valid are
-// the in-range rows / columns of this tile (<= 128).  `lds` holds
-// MP_LDS_WORDS ints.  When C aliases A or B (Floyd-Warshall panels), every
-// read of the aliased operand happens before the epilogue writes C, and the
-// tile is owned by this workgroup alone.
-__device__ __forceinline__ void minplus_tile(const int32_t *A, int64_t lda, const int32_t *B,
+  // Compute the tile.  A points at the tile's first row (k = 0), B at the
+  // tile's first column (k = 0), C at the tile's corner; m_valid / n_valid
+  // are the in-range rows / columns (<= BM / BN).  `lds` holds LDS_WORDS
+  // ints.  When C aliases A or B (Floyd-Warshall panels), every read of the
+  // aliased operand happens before the epilogue writes C, and the tile is
+  // owned by this workgroup alone.
+  __device__ static __forceinline__ void run(const int32_t *A, int64_t lda, const int32_t *B,
                                              int64_t ldb, int32_t *C, int64_t ldc, int m_valid,
                                              int n_valid, int K, bool accumulate, int flags,
                                              int32_t *lds) {
-  int32_t *As = lds;
-  int32_t *Bs = lds + MP_BM * MP_AS_LD;
-  const int t = threadIdx.x;
-  const int wave = t >> 6, lane = t & 63;
-  const int lx = lane & 7, ly = lane >> 3;
-  const int row0 = (wave >> 1) * 64 + ly;        // rows row0 + 8r
-  const int col0 = (wave & 1) * 64 + lx * 4;     // cols col0 + 32h + c
+    int32_t *As = lds;
+    int32_t *Bs = lds + BM * MP_AS_LD;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int lx = lane & 7, ly = lane >> 3;
+    const int row0 = (wave >> 1) * (8 * TM) + ly;    // rows row0 + 8r
+    const int col0 = (wave & 1) * (8 * TN) + lx * 4;  // cols col0 + 32h + c
 
-  int acc[8][8];
+    int acc[TM][TN];
 #pragma unroll
-  for (int r = 0; r < 8; r++)
+    for (int r = 0; r < TM; r++)
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc[r][c] = kInf;
+      for (int c = 0; c < TN; c++) acc[r][c] = kInf;
 
-  int ra[8], rb[8];
-  const int nchunks = (K + MP_BK - 1) / MP_BK;
-  if (nchunks > 0) {
-    mp_load_chunk(A, lda, B, ldb, m_valid, n_valid, K, 0, flags, ra, rb);
-    mp_store_chunk(As, Bs, ra, rb);
-  }
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ch++) {
-    const bool more = ch + 1 < nchunks;
-    if (more) mp_load_chunk(A, lda, B, ldb, m_valid, n_valid, K, (ch + 1) * MP_BK, flags, ra, rb);
-#pragma unroll 1
-    for (int kk = 0; kk < MP_BK; kk += 2) {
-      int a0[8], a1[8], b0[8], b1[8];
-#pragma unroll
-      for (int r = 0; r < 8; r++) {
-        int2 v = *reinterpret_cast<const int2 *>(As + (row0 + 8 * r) * MP_AS_LD + kk);
-        a0[r] = v.x;
-        a1[r] = v.y;
-      }
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        int4 u0 = *reinterpret_cast<const int4 *>(Bs + kk * MP_BS_LD + col0 + 32 * h);
-        int4 u1 = *reinterpret_cast<const int4 *>(Bs + (kk + 1) * MP_BS_LD + col0 + 32 * h);
-        b0[4 * h + 0] = u0.x; b0[4 * h + 1] = u0.y; b0[4 * h + 2] = u0.z; b0[4 * h + 3] = u0.w;
-        b1[4 * h + 0] = u1.x; b1[4 * h + 1] = u1.y; b1[4 * h + 2] = u1.z; b1[4 * h + 3] = u1.w;
-      }
-#pragma unroll
-      for (int r = 0; r < 8; r++)
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-          acc[r][c] = imin(acc[r][c], imin(a0[r] + b0[c], a1[r] + b1[c]));
+    int ra[TM], rb[TN];
+    const int nchunks = (K + MP_BK - 1) / MP_BK;
+    if (nchunks > 0) {
+      load_chunk(A, lda, B, ldb, m_valid, n_valid, K, 0, flags, ra, rb);
+      store_chunk(As, Bs, ra, rb);
     }
     __syncthreads();
-    if (more) {
-      mp_store_chunk(As, Bs, ra, rb);
-      __syncthreads();
-    }
-  }
-
-  // epilogue
+    for (int ch = 0; ch < nchunks; ch++) {
+      const bool more = ch + 1 < nchunks;
+      if (more) load_chunk(A, lda, B, ldb, m_valid, n_valid, K, (ch + 1) * MP_BK, flags, ra, rb);
+#pragma unroll 1
+      for (int kk = 0; kk < MP_BK; kk += 2) {
+        int a0[TM], a1[TM], b0[TN], b1[TN];
 #pragma unroll
-  for (int r = 0; r < 8; r++) {
-    const int row = row0 + 8 * r;
-    if (row >= m_valid) continue;
-    int32_t *crow = C + (int64_t)row * ldc;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int col = col0 + 32 * h;
-      if ((flags & MP_VEC_C) && col + 4 <= n_valid) {
-        int4 *cp = reinterpret_cast<int4 *>(crow + col);
-        int4 v = make_int4(acc[r][4 * h + 0], acc[r][4 * h + 1], acc[r][4 * h + 2],
-                           acc[r][4 * h + 3]);
-        if (accumulate) {
-          int4 o = *cp;
-          v.x = imin(v.x, o.x);
-          v.y = imin(v.y, o.y);
-          v.z = imin(v.z, o.z);
-          v.w = imin(v.w, o.w);
+        for (int r = 0; r < TM; r++) {
+          const int2 v = *reinterpret_cast<const int2 *>(As + (row0 + 8 * r) * MP_AS_LD + kk);
+          a0[r] = v.x;
+          a1[r] = v.y;
         }
-        *cp = v;
-      } else {
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-          if (col + c < n_valid) {
-            int v = acc[r][4 * h + c];
-            if (accumulate) v = imin(v, crow[col + c]);
-            crow[col + c] = v;
+        for (int h = 0; h < TN / 4; h++) {
+          const int4 u0 = *reinterpret_cast<const int4 *>(Bs + kk * BS_LD + col0 + 32 * h);
+          const int4 u1 = *reinterpret_cast<const int4 *>(Bs + (kk + 1) * BS_LD + col0 + 32 * h);
+          b0[4 * h + 0] = u0.x; b0[4 * h + 1] = u0.y; b0[4 * h + 2] = u0.z; b0[4 * h + 3] = u0.w;
+          b1[4 * h + 0] = u1.x; b1[4 * h + 1] = u1.y; b1[4 * h + 2] = u1.z; b1[4 * h + 3] = u1.w;
+        }
+#pragma unroll
+        for (int r = 0; r < TM; r++)
+#pragma unroll
+          for (int c = 0; c < TN; c++)
+            acc[r][c] = imin(acc[r][c], imin(a0[r] + b0[c], a1[r] + b1[c]));
+      }
+      __syncthreads();
+      if (more) {
+        store_chunk(As, Bs, ra, rb);
+        __syncthreads();
+      }
+    }
+
+    // epilogue
+#pragma unroll
+    for (int r = 0; r < TM; r++) {
+      const int row = row0 + 8 * r;
+      if (row >= m_valid) continue;
+      int32_t *crow = C + (int64_t)row * ldc;
+#pragma unroll
+      for (int h = 0; h < TN / 4; h++) {
+        const int col = col0 + 32 * h;
+        if ((flags & MP_VEC_C) && col + 4 <= n_valid) {
+          int4 *cp = reinterpret_cast<int4 *>(crow + col);
+          int4 v = make_int4(acc[r][4 * h + 0], acc[r][4 * h + 1], acc[r][4 * h + 2],
+                             acc[r][4 * h + 3]);
+          if (accumulate) {
+            const int4 o = *cp;
+            v.x = imin(v.x, o.x);
+            v.y = imin(v.y, o.y);
+            v.z = imin(v.z, o.z);
+            v.w = imin(v.w, o.w);
+          }
+          *cp = v;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            if (col + c < n_valid) {
+              int v = acc[r][4 * h + c];
+              if (accumulate) v = imin(v, crow[col + c]);
+              crow[col + c] = v;
+            }
           }
         }
       }
     }
   }
-}
+};
+
+using MpBig = MpTile<8, 8>;    // 128 x 128
+using MpSmall = MpTile<4, 4>;  // 64 x 64
+constexpr int MP_BM = MpBig::BM;
+constexpr int MP_BN = MpBig::BN;
 
 // Map a linear workgroup id to a tile index so that each XCD (workgroups are
 // dealt round-robin over 8 XCDs) works on a contiguous range of tiles, which

@@ -15,7 +15,7 @@
 
 namespace fapi {
 
-int launch_fw(int32_t *d, int32_t n, int64_t ld, hipStream_t s);
+int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s);
 int launch_minplus(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C,
                    int64_t ldc, int32_t M, int32_t N, int32_t K, int accumulate, hipStream_t s);
 int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
@@ -128,7 +128,7 @@ int fap_floyd_path(int num, float *mat, int *path) {
   DevBuf d(ctx);
   FAP_TRY(d.alloc(sizeof(int32_t) * n2));
   FAP_TRY(fap_memcpy_h2d(ctx, d.p, m.data(), sizeof(int32_t) * n2));
-  FAP_TRY(launch_fw((int32_t *)d.p, num, num, ctx->stream));
+  FAP_TRY(launch_fw(ctx, (int32_t *)d.p, num, num, ctx->stream));
   FAP_TRY(fap_memcpy_d2h(ctx, m.data(), d.p, sizeof(int32_t) * n2));
   to_f32(m, mat);
   return FAP_OK;

@@ -1,29 +1,32 @@
-// sssp.hip -- multi-source SSSP (stage 1) by frontier Bellman-Ford.
+// sssp.hip -- multi-source SSSP (stage 1): one workgroup per source,
+// near-far frontier processing, no grid-wide synchronisation.
 //
 // Replaces handle_boundry_path (fap/kernel/batch_sssp.h:254-277), whose CPU
 // branch runs one binary-heap Dijkstra per boundary vertex (:79-114, OpenMP
 // over sources :232-252) and whose HIP branch runs the sources one after
-// another with a host round trip per relaxation round
+// another with a host round trip per Bellman-Ford round
 // (fap/kernel/hip/batch_sssp.cpp:91-158).  Shortest distances are unique, so
-// any exact method gives the reference's rows bit for bit.
+// any exact method reproduces the reference's rows bit for bit.
 //
-// Here all sources of a call advance together.  The frontier is a queue of
-// (source, vertex) items, item = s * V + v (uint32; sources are processed in
-// chunks so that S * V < 2^32 and the per-chunk state fits the budget).
-// Round r: each item (s, u) reads dist[s][u] and relaxes u's out-arcs with
-//   atomicMin(dist[s][v], dist[s][u] + w)     (only when a plain read shows
-//                                              the candidate is smaller)
-// and an improved (s, v) is queued for round r+1 unless already queued for it
-// (atomicMax on a per-(s, v) round stamp).  Queue appends are compacted per
-// wavefront: ballot of the lanes that push, one atomicAdd per wave, lane slot
-// = mbcnt of the ballot.  Arcs are read as packed {col, w} pairs.
+// Method (near-far, Davidson et al. 2014, restated): each workgroup owns one
+// source at a time and keeps, in global memory, two "near" queues (current
+// and next round) and two "far" piles.  A round relaxes the out-arcs of every
+// near vertex u (strict: atomicMin(dist[v], dist[u] + w) only when a plain
+// read shows the candidate is smaller).  An improved v goes to the next near
+// queue if its new distance is below the threshold, otherwise to the far pile.
+// When the near queue drains, the threshold moves to min(far distance) + delta
+// and the far pile is split.  Vertices are therefore settled in near
+// distance order, as Dijkstra's buckets would, and re-relaxations are limited
+// to a band of width delta.
 //
-// The host never waits per round: rounds are launched back to back, each
-// kernel reads its queue length from device memory, and every POLL rounds an
-// asynchronous copy of the next queue length is checked one poll late; the
-// rounds after convergence are empty launches.  Bellman-Ford with
-// non-negative weights settles within V rounds; more rounds than V + 1 is
-// reported as FAP_E_DEVICE.
+// Queue appends are compacted per wavefront: a ballot of the pushing lanes,
+// one LDS atomicAdd per wave, lane slot = mbcnt of the ballot.  Duplicates
+// are suppressed by a per-vertex round stamp (near) and a per-source tag
+// (far); both arrays are never cleared, the stamps only grow.
+//
+// Every loop is bounded: a vertex enters a queue only after a strict
+// improvement of its distance, and each far split raises the threshold above
+// the smallest far distance.  Workgroups never wait on each other.
 #include <algorithm>
 #include <vector>
 
@@ -31,83 +34,171 @@
 
 namespace fapk {
 
-constexpr int SSSP_THREADS = 256;
 
-__global__ __launch_bounds__(SSSP_THREADS) void sssp_init_kernel(
-    int32_t *dist, int64_t ld, int V, int S, const int32_t *sources, int32_t *stamp,
-    uint32_t *q0, uint32_t *counts) {
-  const int64_t total = (int64_t)S * V;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int s = (int)(i / V);
-    const int v = (int)(i - (int64_t)s * V);
-    dist[(int64_t)s * ld + v] = (v == sources[s]) ? 0 : 0x3FFFFFFF;
-    stamp[i] = 0;
-  }
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += stride)
-    q0[s] = (uint32_t)(s * V + sources[s]);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    counts[0] = (uint32_t)S;
-    counts[1] = 0;
-    counts[2] = 0;
-  }
-}
+struct NfState {
+  uint32_t *qa, *qb, *fa, *fb;  // near current/next, far current/next (V each)
+  int32_t *stamp;               // near round stamp (V)
+  int32_t *farmark;             // far-pile tag (V)
+};
 
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-__global__ __launch_bounds__(SSSP_THREADS) void sssp_round_kernel(
-    const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int32_t *dist,
-    int64_t ld, int V, const uint32_t *__restrict__ qcur, uint32_t *__restrict__ qnext,
-    uint32_t *counts, int32_t *stamp, int r) {
-  const uint32_t n = counts[r % 3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) counts[(r + 2) % 3] = 0;
-  uint32_t *next_count = counts + (r + 1) % 3;
-  const int stamp_val = r + 1;
+// Append `item` for lanes with `push`; `cnt` is an LDS counter.
+__device__ __forceinline__ void wave_push(bool push, uint32_t item, uint32_t *q, uint32_t *cnt) {
+  const unsigned long long mask = __ballot(push);
+  if (!mask) return;
   const int lane = threadIdx.x & 63;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    const uint32_t idx = base + threadIdx.x;
-    const bool valid = idx < n;
-    int s = 0, u = 0, du = 0, e = 0, e_end = 0;
-    if (valid) {
-      const uint32_t item = qcur[idx];
-      s = (int)(item / (uint32_t)V);
-      u = (int)(item - (uint32_t)s * (uint32_t)V);
-      du = dist[(int64_t)s * ld + u];
-      e = row_offset[u];
-      e_end = row_offset[u + 1];
-    }
-    while (__any(valid && e < e_end)) {
-      bool push = false;
-      uint32_t pitem = 0;
-      if (valid && e < e_end) {
-        const int2 ed = edges[e];
-        e++;
-        const int nd = du + ed.y;
-        int32_t *dp = dist + (int64_t)s * ld + ed.x;
-        if (nd < *dp) {
-          const int old = atomicMin(dp, nd);
-          if (nd < old) {
-            const uint32_t key = (uint32_t)s * (uint32_t)V + (uint32_t)ed.x;
-            if (atomicMax(stamp + key, stamp_val) < stamp_val) {
-              push = true;
-              pitem = key;
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (push) q[base + lane_prefix(mask)] = item;
+}
+
+template <int NF_THREADS>
+__global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
+    const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
+    const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld, uint32_t *ws,
+    int64_t ws_stride, int delta) {
+  // Queue counters live in LDS in three sets of {next near, new far}.  Each
+  // phase (a near round or a far split) appends into set ci, thread 0 zeroes
+  // set ci+1 (mod 3) -- last read before the previous barrier, next written
+  // after this one -- then one barrier, then every thread reads set ci.
+  __shared__ uint32_t s_cnt[6];
+  __shared__ int s_min;
+  uint32_t *base = ws + (int64_t)blockIdx.x * ws_stride;
+  NfState st;
+  st.qa = base;
+  st.qb = base + V;
+  st.fa = base + 2 * (int64_t)V;
+  st.fb = base + 3 * (int64_t)V;
+  st.stamp = (int32_t *)(base + 4 * (int64_t)V);
+  st.farmark = (int32_t *)(base + 5 * (int64_t)V);
+  const int tid = threadIdx.x;
+  // stamps are per workgroup and only grow: start from 0 once per launch
+  for (int v = tid; v < V; v += NF_THREADS) {
+    st.stamp[v] = 0;
+    st.farmark[v] = -1;
+  }
+  if (tid < 6) s_cnt[tid] = 0;
+  int round_tag = 0;
+  int ci = 0;
+  for (int si = blockIdx.x; si < S; si += gridDim.x) {
+    int32_t *dist = out + (int64_t)si * ld;
+    const int src = sources[si];
+    for (int v = tid; v < V; v += NF_THREADS) dist[v] = (v == src) ? 0 : 0x3FFFFFFF;
+    uint32_t *qa = st.qa, *qb = st.qb, *fa = st.fa, *fb = st.fb;
+    uint32_t n_near = 1, n_far = 0;
+    if (tid == 0) qa[0] = (uint32_t)src;
+    int thr = delta;
+    const int tag = si;  // distinct per source handled by this workgroup
+    __syncthreads();
+    for (;;) {
+      // ---- drain the near queue, one barrier per round ----
+      while (n_near > 0) {
+        round_tag++;
+        uint32_t *cn = &s_cnt[2 * ci];
+        for (uint32_t i0 = 0; i0 < n_near; i0 += NF_THREADS) {
+          const uint32_t i = i0 + tid;
+          const bool valid = i < n_near;
+          int du = 0, e = 0, e_end = 0;
+          if (valid) {
+            const int u = (int)qa[i];
+            du = dist[u];
+            e = row_offset[u];
+            e_end = row_offset[u + 1];
+          }
+          while (__any(valid && e < e_end)) {
+            bool pn = false, pf = false;
+            uint32_t item = 0;
+            if (valid && e < e_end) {
+              const int2 ed = edges[e];
+              e++;
+              const int nd = du + ed.y;
+              int32_t *dp = dist + ed.x;
+              if (nd < *dp) {
+                const int old = atomicMin(dp, nd);
+                if (nd < old) {
+                  item = (uint32_t)ed.x;
+                  if (nd < thr) {
+                    pn = atomicMax(st.stamp + ed.x, round_tag) < round_tag;
+                  } else {
+                    pf = atomicExch(st.farmark + ed.x, tag) != tag;
+                  }
+                }
+              }
+            }
+            wave_push(pn, item, qb, &cn[0]);
+            wave_push(pf, item, fa + n_far, &cn[1]);
+          }
+        }
+        const int cz = ci == 2 ? 0 : ci + 1;
+        if (tid == 0) {
+          s_cnt[2 * cz] = 0;
+          s_cnt[2 * cz + 1] = 0;
+        }
+        __syncthreads();
+        n_near = cn[0];
+        n_far += cn[1];
+        ci = cz;
+        uint32_t *t = qa;
+        qa = qb;
+        qb = t;
+      }
+      if (n_far == 0) break;
+      // ---- threshold = min far distance + delta ----
+      if (tid == 0) s_min = 0x7FFFFFFF;
+      __syncthreads();
+      int lmin = 0x7FFFFFFF;
+      for (uint32_t i = tid; i < n_far; i += NF_THREADS) {
+        const int v = (int)fa[i];
+        if (st.farmark[v] == tag) lmin = min(lmin, dist[v]);
+      }
+      for (int off = 32; off > 0; off >>= 1) lmin = min(lmin, __shfl_down(lmin, off));
+      if ((tid & 63) == 0) atomicMin(&s_min, lmin);
+      __syncthreads();
+      const int fmin = s_min;
+      if (fmin == 0x7FFFFFFF) break;  // every far entry was stale
+      thr = fmin + delta;
+      round_tag++;
+      // ---- split the far pile: below thr -> near, else stays far ----
+      uint32_t *cn = &s_cnt[2 * ci];
+      for (uint32_t i0 = 0; i0 < n_far; i0 += NF_THREADS) {
+        const uint32_t i = i0 + tid;
+        bool pn = false, pf = false;
+        uint32_t item = 0;
+        if (i < n_far) {
+          const int v = (int)fa[i];
+          item = (uint32_t)v;
+          if (st.farmark[v] == tag) {
+            if (dist[v] < thr) {
+              st.farmark[v] = -1;
+              pn = atomicMax(st.stamp + v, round_tag) < round_tag;
+            } else {
+              pf = true;
             }
           }
         }
+        wave_push(pn, item, qa, &cn[0]);
+        wave_push(pf, item, fb, &cn[1]);
       }
-      const unsigned long long mask = __ballot(push);
-      if (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        uint32_t wbase = 0;
-        if (lane == leader) wbase = atomicAdd(next_count, (uint32_t)__popcll(mask));
-        wbase = __shfl(wbase, leader);
-        if (push) qnext[wbase + lane_prefix(mask)] = pitem;
+      const int cz = ci == 2 ? 0 : ci + 1;
+      if (tid == 0) {
+        s_cnt[2 * cz] = 0;
+        s_cnt[2 * cz + 1] = 0;
       }
+      __syncthreads();
+      n_near = cn[0];
+      n_far = cn[1];
+      ci = cz;
+      uint32_t *t = fa;
+      fa = fb;
+      fb = t;
     }
+    // far piles are reused by the next source: tags make stale marks harmless
   }
 }
 
@@ -115,45 +206,13 @@ __global__ __launch_bounds__(SSSP_THREADS) void sssp_round_kernel(
 
 namespace fapi {
 
-static constexpr int kPoll = 16;
-
-// Runs one chunk of sources (S * V < 2^32, state already sized).
-static int sssp_chunk(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int S,
-                      int32_t *d_out, int64_t ld, hipStream_t s, uint32_t *q0, uint32_t *q1,
-                      int32_t *stamp, uint32_t *counts, hipEvent_t ev[2]) {
-  const int V = g->V;
-  const int64_t total = (int64_t)S * V;
-  int64_t blocks = std::min<int64_t>((total + fapk::SSSP_THREADS - 1) / fapk::SSSP_THREADS,
-                                     (int64_t)ctx->num_cus * 16);
-  blocks = std::max<int64_t>(blocks, 1);
-  hipLaunchKernelGGL(fapk::sssp_init_kernel, dim3((unsigned)blocks), dim3(fapk::SSSP_THREADS),
-                     0, s, d_out, ld, V, S, d_sources, stamp, q0, counts);
-  FAP_CHECK_LAUNCH();
-  const unsigned round_blocks = (unsigned)std::max(1, ctx->num_cus * 8);
-  uint32_t *slot = ctx->host_pinned;  // two poll slots
-  const int max_rounds = V + 2;
-  int pending = -1;  // poll index whose event is outstanding
-  for (int r = 0;; r++) {
-    if (r > max_rounds)
-      return fail(FAP_E_DEVICE, "sssp: no convergence within V + 1 rounds");
-    uint32_t *qc = (r & 1) ? q1 : q0;
-    uint32_t *qn = (r & 1) ? q0 : q1;
-    hipLaunchKernelGGL(fapk::sssp_round_kernel, dim3(round_blocks), dim3(fapk::SSSP_THREADS),
-                       0, s, g->row_offset, g->edges, d_out, ld, V, qc, qn, counts, stamp, r);
-    FAP_CHECK_LAUNCH();
-    if ((r + 1) % kPoll == 0) {
-      const int p = ((r + 1) / kPoll) & 1;
-      FAP_HIP(hipMemcpyAsync(slot + p, counts + (r + 1) % 3, sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, s));
-      FAP_HIP(hipEventRecord(ev[p], s));
-      if (pending >= 0) {
-        FAP_HIP(hipEventSynchronize(ev[pending]));
-        if (slot[pending] == 0) break;
-      }
-      pending = p;
-    }
-  }
-  return FAP_OK;
+// Threshold width: 8 mean arc weights, the best of 1..32 x mean measured on
+// the 512 x 512 road-like grid (w 1..100) and the 64 x 64 unit grid
+// (bench.py --workload ssspsweep).
+static int pick_delta(const fap_graph_dev *g) {
+  if (g->E == 0) return 1;
+  const double mean = (double)g->weight_sum / (double)g->E;
+  return (int)std::max(1.0, std::min(1.0e6, 8.0 * mean));
 }
 
 int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
@@ -161,34 +220,34 @@ int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int
   if (n_src <= 0) return FAP_OK;
   const int V = g->V;
   if (V <= 0) return fail(FAP_E_ARG, "sssp: empty graph");
-  // chunk so that S*V < 2^32 and the queue+stamp state stays within ~4 GiB
-  const int64_t by_index = (int64_t)(UINT32_MAX - 1) / V;
-  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)4 << 30) / (12 * (int64_t)V));
-  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({by_index, by_mem, n_src}));
-  const size_t cells = (size_t)chunk * V;
-  FAP_TRY(ensure_ws(ctx, ctx->ws_queue, 2 * cells * sizeof(uint32_t)));
-  FAP_TRY(ensure_ws(ctx, ctx->ws_stamp, cells * sizeof(int32_t)));
-  FAP_TRY(ensure_ws(ctx, ctx->ws_small, 64));
-  uint32_t *q0 = (uint32_t *)ctx->ws_queue.ptr;
-  uint32_t *q1 = q0 + cells;
-  int32_t *stamp = (int32_t *)ctx->ws_stamp.ptr;
-  uint32_t *counts = (uint32_t *)ctx->ws_small.ptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  FAP_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  hipError_t he = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
-  if (he != hipSuccess) {
-    (void)hipEventDestroy(ev[0]);
-    return hip_fail(he, "hipEventCreate", __FILE__, __LINE__);
+  // per-workgroup state: 6 V words; as many workgroups as sources, up to
+  // 2 per CU and a 4 GiB state budget
+  const int64_t stride = 6 * (int64_t)V;
+  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)4 << 30) / (stride * 4));
+  const int per_cu = ctx->opt_sssp_threads == 256 ? 4 : ctx->opt_sssp_threads == 512 ? 2 : 1;
+  const int grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)n_src, (int64_t)ctx->num_cus * per_cu, by_mem}));
+  FAP_TRY(ensure_ws(ctx, ctx->ws_queue, (size_t)grid * stride * sizeof(uint32_t)));
+  const int delta = ctx->opt_sssp_delta > 0 ? ctx->opt_sssp_delta
+                    : g->delta > 0          ? g->delta
+                                            : pick_delta(g);
+  uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
+  switch (ctx->opt_sssp_threads) {
+    case 256:
+      hipLaunchKernelGGL(fapk::sssp_nf_kernel<256>, dim3(grid), dim3(256), 0, s, g->row_offset,
+                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+      break;
+    case 512:
+      hipLaunchKernelGGL(fapk::sssp_nf_kernel<512>, dim3(grid), dim3(512), 0, s, g->row_offset,
+                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+      break;
+    default:
+      hipLaunchKernelGGL(fapk::sssp_nf_kernel<1024>, dim3(grid), dim3(1024), 0, s, g->row_offset,
+                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+      break;
   }
-  int rc = FAP_OK;
-  for (int32_t s0 = 0; s0 < n_src && rc == FAP_OK; s0 += chunk) {
-    const int S = std::min<int32_t>(chunk, n_src - s0);
-    rc = sssp_chunk(ctx, g, d_sources + s0, S, d_out + (int64_t)s0 * ld_out, ld_out, s, q0, q1,
-                    stamp, counts, ev);
-  }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  return rc;
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
 }
 
 }  // namespace fapi
@@ -212,4 +271,10 @@ extern "C" int fap_sssp_multi(fap_ctx *ctx, const fap_graph_dev *g, const int32_
     if (h[i] < 0 || h[i] >= g->V)
       return fail(FAP_E_ARG, "sssp: source " + std::to_string(i) + " out of range");
   return run_sssp(ctx, g, d_sources, n_src, d_out, ld_out, s);
+}
+
+extern "C" int fap_graph_set_delta(fap_graph_dev *g, int32_t delta) {
+  if (!g || delta < 0) return fail(FAP_E_ARG, "bad graph/delta");
+  g->delta = delta;
+  return FAP_OK;
 }

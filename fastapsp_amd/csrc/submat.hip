@@ -75,7 +75,7 @@ namespace fapi {
 
 int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
              int32_t *d_out, int64_t ld_out, hipStream_t s);
-int launch_fw(int32_t *d, int32_t n, int64_t ld, hipStream_t s);
+int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s);
 int launch_minplus(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C,
                    int64_t ldc, int32_t M, int32_t N, int32_t K, int accumulate,
                    hipStream_t s);
@@ -130,7 +130,7 @@ int run_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int3
   FAP_TRY(run_sssp(ctx, g, g->st2ed + start, n_bdy, rowblock, ld_rb, s));
   // stage 2: dense part matrix and its closure
   FAP_TRY(launch_gather(g, start, n_sub, n_bdy, rowblock, ld_rb, sub, ld_sub, s));
-  FAP_TRY(launch_fw(sub, n_sub, ld_sub, s));
+  FAP_TRY(launch_fw(ctx, sub, n_sub, ld_sub, s));
   // stage 3: inner rows = (inner x bdy) (x) (bdy x V), starting from INF
   const int32_t inner = n_sub - n_bdy;
   FAP_TRY(launch_minplus(sub + (int64_t)n_bdy * ld_sub, ld_sub, rowblock, ld_rb,

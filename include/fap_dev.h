@@ -61,6 +61,13 @@ int fap_device_count(int *count);
 int fap_ctx_create(int device, fap_ctx **out);
 int fap_ctx_destroy(fap_ctx *ctx);
 int fap_ctx_stream(fap_ctx *ctx, void **stream);
+/* Tuning options; they change speed, never results:
+ *   "fw_lookahead" 0/1      overlap FW round r+1's diagonal/panels with round r
+ *   "fw_tile"      0/64/128 FW block size (0 = 64)
+ *   "sssp_threads" 256/512/1024  threads per SSSP workgroup (one source each)
+ *   "sssp_delta"   >= 0     near-far band width (0 = per graph, 8 x mean weight) */
+int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
+int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);
 int fap_dev_alloc(fap_ctx *ctx, size_t bytes, void **dptr);
 int fap_dev_free(fap_ctx *ctx, void *dptr);
@@ -96,6 +103,9 @@ int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g,
                             const int32_t *graph_id, const int32_t *st2ed,
                             const int32_t *ed2st);
 int fap_graph_free(fap_ctx *ctx, fap_graph_dev *g);
+/* Near-far band width for fap_sssp_multi on this graph (0 = 8 x mean arc
+ * weight).  Affects speed only, never the distances. */
+int fap_graph_set_delta(fap_graph_dev *g, int32_t delta);
 int fap_graph_info(const fap_graph_dev *g, int32_t *V, int64_t *E);
 
 /* ---- the three stages ---- */
@@ -110,6 +120,32 @@ int fap_sssp_multi(fap_ctx *ctx, const fap_graph_dev *g,
  * blocked: diagonal tile in LDS, row/column panels, remaining tiles. */
 int fap_fw_blocked(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t ld,
                    void *stream);
+
+/* Closure of `count` matrices (d_mats[j]: n[j] x n[j], stride ld[j]; host
+ * arrays of device pointers and sizes) together: 64 x 64 tiles, one launch
+ * per phase across all matrices (diag / panels / rest of round r), each a
+ * persistent grid over a size-ordered task queue.  Same results as
+ * fap_fw_blocked on each matrix.  Synchronous. */
+int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                   const int64_t *ld, void *stream);
+
+/* A reusable batch: descriptors uploaded once; with use_graph the phase
+ * launches are captured in a hipGraph and each run replays it (the
+ * matrices are read and written in place at every run). */
+typedef struct fap_fw_plan fap_fw_plan;
+int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                       const int64_t *ld, int use_graph, fap_fw_plan **out);
+int fap_fw_plan_run(fap_fw_plan *plan, void *stream);
+int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks, int64_t *ops);
+int fap_fw_plan_destroy(fap_fw_plan *plan);
+
+/* Dense distance matrix of each whole graph (K = 1 part): INF, then every
+ * arc in CSR order (last duplicate wins), diagonal 0 -- what
+ * MysubMatBuild_path builds when a part is the whole graph
+ * (fap/utils/util_Myalgorithm.h:30-73).  d_mats[j] is V_j x V_j, stride
+ * ld[j].  Two launches for the whole batch; synchronous. */
+int fap_dense_from_graphs(fap_ctx *ctx, int32_t count, const fap_graph_dev *const *graphs,
+                          int32_t *const *d_mats, const int64_t *ld, void *stream);
 
 /* Min-plus product: C[i][j] = min(base, min_k A[i][k] + B[k][j]) where base
  * is C[i][j] if accumulate else INF.  A is M x K (lda), B is K x N (ldb),

@@ -110,8 +110,22 @@ def test_minplus_rejects_bad_arguments(gpu_ctx):
 
 # ---------------- Floyd-Warshall ----------------
 
-@pytest.mark.parametrize("n", [1, 2, 5, 64, 127, 128, 129, 300, 513])
-def test_fw_matches_oracle(gpu_ctx, n):
+@pytest.fixture
+def fw_config(gpu_ctx, request):
+    tile, look = request.param
+    gpu_ctx.set_option("fw_tile", tile)
+    gpu_ctx.set_option("fw_lookahead", look)
+    yield request.param
+    gpu_ctx.set_option("fw_tile", 0)
+    gpu_ctx.set_option("fw_lookahead", 0)
+
+
+FW_CONFIGS = [(64, 0), (128, 0), (64, 1), (128, 1)]
+
+
+@pytest.mark.parametrize("fw_config", FW_CONFIGS, indirect=True, ids=lambda c: f"T{c[0]}-la{c[1]}")
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 128, 129, 200, 300, 513])
+def test_fw_matches_oracle(gpu_ctx, fw_config, n):
     m = synth.dense_matrix(n, min(1.0, 6.0 / n), 1, 1000, n)
     d = gpu_ctx.to_device(m)
     gpu_ctx.fw(d.ptr, n, n)
@@ -129,7 +143,8 @@ def test_fw_golden_and_strided(gpu_ctx):
         assert (got[:, 70:] == -7).all()
 
 
-def test_fw_positive_diagonal_and_zero_weights(gpu_ctx):
+@pytest.mark.parametrize("fw_config", FW_CONFIGS[:2], indirect=True, ids=lambda c: f"T{c[0]}")
+def test_fw_positive_diagonal_and_zero_weights(gpu_ctx, fw_config):
     n = 200
     m = synth.dense_matrix(n, 0.03, 0, 50, 77)
     rng = np.random.default_rng(3)
@@ -155,8 +170,20 @@ def test_sssp_golden_all_sources(gpu_ctx, name):
     dg.close()
 
 
-def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx):
-    u, v, w = synth.grid_graph(64, 64, 0.05, 0.0, 1, 1, 1)
+@pytest.fixture
+def sssp_config(gpu_ctx, request):
+    threads, delta = request.param
+    gpu_ctx.set_option("sssp_threads", threads)
+    gpu_ctx.set_option("sssp_delta", delta)
+    yield request.param
+    gpu_ctx.set_option("sssp_threads", 1024)
+    gpu_ctx.set_option("sssp_delta", 0)
+
+
+@pytest.mark.parametrize("sssp_config", [(1024, 0), (256, 1), (512, 7), (1024, 100000)],
+                         indirect=True, ids=lambda c: f"t{c[0]}-d{c[1]}")
+def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
+    u, v, w = synth.grid_graph(64, 64, 0.05, 0.02, 1, 60, 1)
     V = 4096
     ro, col, wt = synth.to_csr(V, u, v, w, False)
     dg = gpu_ctx.upload_graph(ro, col, wt)
