@@ -292,6 +292,87 @@ def bench_grid_pipeline(args):
     return 0
 
 
+def bench_grid_batch(args):
+    """Plain vs batched per-part solve on the 512x512 road-like grid with
+    K=256: groups of --group parts, each group either part after part
+    (fap_subgraph_solve) or in one batched call (fap_subgraph_solve_batch:
+    one SSSP launch for every boundary vertex of the group, one batched
+    closure, then the per-part products).  Row blocks are checksummed on the
+    device after each group (outside the timed region) and must agree."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    R, K, G = args.grid, args.parts_k, args.group
+    V = R * R
+    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    E = len(col)
+    g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
+    g.preCondition()
+    vn, vo, bn, _ = g.part_info()
+    st2ed = g.getMapping()
+    gid = g.getGraphId()
+    ed2st = np.empty(V, np.int32)
+    ed2st[st2ed] = np.arange(V, dtype=np.int32)
+    g.close()
+    ctx = fa.Context(0)
+    dg = ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, st2ed, ed2st)
+    parts = list(range(1, K + 1))[: args.max_parts] if args.max_parts else list(range(1, K + 1))
+    groups = [parts[i:i + G] for i in range(0, len(parts), G)]
+    nmax = int(vn[1:].max())
+    blks = [ctx.alloc((nmax, V)) for _ in range(min(G, len(parts)))]
+    log(f"grid-batch: V={V} E={E} K={K} parts={len(parts)} group={G} "
+        f"blocks={len(blks)}x{nmax}x{V}")
+
+    def run(batched):
+        t = 0.0
+        sums = []
+        for grp in groups:
+            ctx.sync()
+            t0 = time.perf_counter()
+            if batched:
+                dg.subgraph_solve_batch([int(vo[p]) for p in grp], [int(vn[p]) for p in grp],
+                                        [int(bn[p]) for p in grp],
+                                        [blks[i].ptr for i in range(len(grp))], V)
+            else:
+                for i, p in enumerate(grp):
+                    dg.subgraph_solve(int(vo[p]), int(vn[p]), int(bn[p]), blks[i].ptr, V)
+            ctx.sync()
+            t += time.perf_counter() - t0
+            sums += [ctx.checksum(blks[i].ptr, int(vn[p]) * V) for i, p in enumerate(grp)]
+        return t, sums
+
+    run(True)  # warm-up (allocates workspaces, loads kernels)
+    run(False)
+    res = {}
+    for mode in ("plain", "batched"):
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t, sums = run(mode == "batched")
+            ts.append(t)
+        res[mode] = (min(ts), sums)
+        log(f"  {mode}: {1e3 * min(ts):.1f} ms for {len(parts)} parts")
+    ops = sum(2.0 * int(vn[p]) ** 3 + 2.0 * (int(vn[p]) - int(bn[p])) * int(bn[p]) * V
+              for p in parts)
+    tp, sp = res["plain"]
+    tb, sb = res["batched"]
+    out = {
+        "metric": "min-plus GOp/s", "value": round(ops / tb / 1e9, 1), "unit": "GOp/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * tb, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{R}x{R} grid (-10% edges, +2% diagonals, w 1..100), K={K}, "
+                               f"batched per-part solve, groups of {G}", "V": V, "E": E,
+                   "K": K, "parts": len(parts), "group": G},
+        "plain_ms": round(1e3 * tp, 2), "batched_ms": round(1e3 * tb, 2),
+        "batched_over_plain": round(tp / tb, 2),
+        "plain_GOp_s": round(ops / tp / 1e9, 1),
+        "identical": bool(sp == sb),
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if sp == sb else 1
+
+
 def _time_ms(lib, ctx, fn, reps=2):
     """Average device milliseconds of fn() on the context stream."""
     import fastapsp_amd as fa
@@ -470,21 +551,25 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-rows", type=int, default=24)
     ap.add_argument("--workload", default="dense",
-                    choices=["dense", "copy", "grid", "fwsweep", "ssspsweep", "batch"])
+                    choices=["dense", "copy", "grid", "gridbatch", "fwsweep", "ssspsweep",
+                             "batch"])
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--sizes", default="256,512,1024,2048,4096,8192")
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--parts-k", type=int, default=256)
     ap.add_argument("--max-parts", type=int, default=0)
+    ap.add_argument("--group", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the parts of a "
-              "graph shard across GPUs without exchange, see DESIGN.md", file=sys.stderr)
+              "graph shard across GPUs without exchange", file=sys.stderr)
         return 2
     if args.workload == "copy":
         return bench_copy(args)
     if args.workload == "grid":
         return bench_grid_pipeline(args)
+    if args.workload == "gridbatch":
+        return bench_grid_batch(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "fwsweep":

@@ -74,6 +74,8 @@ _SIGS = {
     "fap_gather_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fap_scatter_submat": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fap_subgraph_solve": (_int, [_vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp]),
+    "fap_subgraph_solve_batch": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, C.POINTER(_vp),
+                                        _i64, _vp]),
     # fap_ops.h
     "fap_handle_boundry_path": (_int, [_pf32, _pi32, _int, _int, _int, _pi32, _pi32, _pi32,
                                        _pf32, _pi32, _int]),
@@ -88,6 +90,8 @@ _SIGS = {
     "fapg_is_split": (_int, [_vp, C.POINTER(_int)]),
     "fapg_solve": (_int, [_vp, _int]),
     "fapg_solve_subgraph": (_int, [_vp, _i32, _int]),
+    "fapg_set_batched": (_int, [_vp, _int]),
+    "fapg_solve_subgraphs": (_int, [_vp, _i32, _pi32, _int]),
     "fapg_num_vertices": (_int, [_vp, _pi32]),
     "fapg_num_edges": (_int, [_vp, C.POINTER(_i64)]),
     "fapg_num_parts": (_int, [_vp, _pi32]),
@@ -346,6 +350,16 @@ class DeviceGraph:
     def subgraph_solve(self, start, n_sub, n_bdy, rowblock, ld_rb, stream=None):
         check(load().fap_subgraph_solve(self.ctx.handle, self.handle, start, n_sub, n_bdy,
                                         rowblock, ld_rb, stream), "fap_subgraph_solve")
+
+    def subgraph_solve_batch(self, starts, n_subs, n_bdys, rowblocks, ld_rb, stream=None):
+        k = len(starts)
+        S = np.ascontiguousarray(starts, dtype=np.int32)
+        N = np.ascontiguousarray(n_subs, dtype=np.int32)
+        B = np.ascontiguousarray(n_bdys, dtype=np.int32)
+        R = (_vp * k)(*rowblocks)
+        check(load().fap_subgraph_solve_batch(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
+                                              _p32(B), R, ld_rb, stream),
+              "fap_subgraph_solve_batch")
 
     def close(self):
         if self.handle:

@@ -112,6 +112,20 @@ int fapg_solve_subgraph(fapg *h, int32_t id, int is_path_needed) {
   return FAP_OK;
 }
 
+int fapg_set_batched(fapg *h, int batched) {
+  FAPG_CHECK(h);
+  h->g->setBatched(batched != 0);
+  return FAP_OK;
+}
+
+int fapg_solve_subgraphs(fapg *h, int32_t count, const int32_t *ids, int is_path_needed) {
+  FAPG_CHECK(h);
+  if (count < 0 || (count > 0 && !ids)) return fail(FAP_E_ARG, "bad id list");
+  std::vector<int32_t> v(ids, ids + count);
+  if (h->g->solveSubGraphs(v, is_path_needed != 0) != 0) return from_status(h->g);
+  return FAP_OK;
+}
+
 int fapg_num_vertices(fapg *h, int32_t *V) {
   FAPG_CHECK(h);
   if (!V) return fail(FAP_E_ARG, "null output");

@@ -59,6 +59,8 @@ struct fap_graph_dev {
   int32_t max_degree = 0;
   int64_t weight_sum = 0;
   int32_t delta = 0;  // SSSP near-far band width (0 = derived from the mean weight)
+  std::vector<int32_t> h_st2ed;     // host copy of st2ed (source lists of batched solves)
+  std::vector<int32_t> h_graph_id;  // host copy of graph_id (part range checks)
 };
 
 struct fap_ctx {
@@ -75,6 +77,7 @@ struct fap_ctx {
   fapi::Workspace ws_misc;
   fapi::Workspace ws_batch;   // batched-closure descriptors and counters
   fapi::Workspace ws_dense;   // dense-build descriptors
+  fapi::Workspace ws_srcs;    // batched-solve sources and row pointers
   uint32_t *host_pinned = nullptr;  // small pinned host buffer for counters
   hipEvent_t ev_poll = nullptr;
   // second stream and events for the Floyd-Warshall look-ahead

@@ -169,7 +169,7 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   Workspace *all[] = {&ctx->ws_sub, &ctx->ws_queue, &ctx->ws_stamp, &ctx->ws_small,
-                      &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense};
+                      &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense, &ctx->ws_srcs};
   for (Workspace *w : all)
     if (w->ptr) (void)hipFree(w->ptr);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
@@ -428,6 +428,8 @@ int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g, const int32_t *graph
                              ctx->stream));
   }
   FAP_HIP(hipStreamSynchronize(ctx->stream));
+  g->h_st2ed.assign(st2ed, st2ed + V);
+  g->h_graph_id.assign(graph_id, graph_id + V);
   return FAP_OK;
 }
 

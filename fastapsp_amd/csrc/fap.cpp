@@ -39,11 +39,16 @@ struct fapGraph::Device {
   std::vector<int32_t> host_block;  // int32 copy of d_block
   bool all_host_valid = false;
   std::vector<int32_t> host_all;
+  // solveSubGraphs: one buffer, one block per part id
+  int32_t *d_multi = nullptr;
+  size_t multi_bytes = 0;
+  std::unordered_map<int32_t, int64_t> multi_off;  // part id -> row offset in d_multi
 
   ~Device() {
     if (ctx) {
       if (d_block) fap_dev_free(ctx, d_block);
       if (d_all) fap_dev_free(ctx, d_all);
+      if (d_multi) fap_dev_free(ctx, d_multi);
       if (g) fap_graph_free(ctx, g);
       fap_ctx_destroy(ctx);
     }
@@ -197,6 +202,7 @@ int32_t fapGraph::setPartition(const std::vector<int32_t> &gid) {
   if (dev_) {
     dev_->partition_uploaded = false;
     dev_->block_id = 0;
+    dev_->multi_off.clear();
   }
   used_metis_ = false;
   loaded_ = true;
@@ -241,6 +247,7 @@ void fapGraph::preCondition() {
   if (dev_) {
     dev_->partition_uploaded = false;
     dev_->block_id = 0;
+    dev_->multi_off.clear();
   }
 }
 
@@ -294,6 +301,25 @@ int32_t fapGraph::run(int32_t id, int32_t *d_rowblock) {
   return 0;
 }
 
+// Batched form of run() over several parts (fap_subgraph_solve_batch).
+int32_t fapGraph::run_batch(const std::vector<int32_t> &ids, const std::vector<int32_t *> &blocks) {
+  std::vector<int32_t> starts, ns, bs;
+  std::vector<int32_t *> rb;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const int32_t id = ids[i];
+    if (C_BlockVer_num[id] == 0) continue;
+    starts.push_back(C_BlockVer_offset[id]);
+    ns.push_back(C_BlockVer_num[id]);
+    bs.push_back(C_BlockBdy_num[id]);
+    rb.push_back(blocks[i]);
+  }
+  if (starts.empty()) return 0;
+  if (fap_subgraph_solve_batch(dev_->ctx, dev_->g, (int32_t)starts.size(), starts.data(),
+                               ns.data(), bs.data(), rb.data(), num_vertexs, nullptr) != FAP_OK)
+    return fail(FAP_E_HIP, std::string("batched parts: ") + fap_last_error());
+  return 0;
+}
+
 // fap/fap.cpp:184-194.  The full V x V matrix, parts stacked in part order.
 int32_t fapGraph::solve(bool is_path_needed_) {
   reset_status();
@@ -312,8 +338,18 @@ int32_t fapGraph::solve(bool is_path_needed_) {
     d.d_all = (int32_t *)p;
     d.all_bytes = bytes;
   }
-  for (int32_t i = 1; i <= K; i++) {
-    if (run(i, d.d_all + (int64_t)C_BlockVer_offset[i] * num_vertexs) != 0) return -1;
+  if (batched_) {
+    std::vector<int32_t> ids;
+    std::vector<int32_t *> blocks;
+    for (int32_t i = 1; i <= K; i++) {
+      ids.push_back(i);
+      blocks.push_back(d.d_all + (int64_t)C_BlockVer_offset[i] * num_vertexs);
+    }
+    if (run_batch(ids, blocks) != 0) return -1;
+  } else {
+    for (int32_t i = 1; i <= K; i++) {
+      if (run(i, d.d_all + (int64_t)C_BlockVer_offset[i] * num_vertexs) != 0) return -1;
+    }
   }
   if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
   full_solved_ = true;
@@ -351,6 +387,51 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool is_path_needed_) {
   current_subgraph_id = id;
   status_ = 0;
   error_.clear();
+  if (is_path_needed) error_ = "path matrices are not produced (is_path_needed ignored)";
+  return 0;
+}
+
+int32_t fapGraph::solveSubGraphs(const std::vector<int32_t> &ids, bool is_path_needed_) {
+  reset_status();
+  is_path_needed = is_path_needed_;
+  for (int32_t id : ids)
+    if (!part_ok(id)) return fail(FAP_E_ARG, "part id outside 1..K or preCondition() not run");
+  if (ensure_device() != 0) return -1;
+  Device &d = *dev_;
+  int64_t rows = 0;
+  std::unordered_map<int32_t, int64_t> off;
+  for (int32_t id : ids) {
+    if (off.count(id)) continue;
+    off[id] = rows;
+    rows += C_BlockVer_num[id];
+  }
+  const size_t bytes = (size_t)std::max<int64_t>(rows, 1) * num_vertexs * sizeof(int32_t);
+  if (d.multi_bytes < bytes) {
+    if (d.d_multi) fap_dev_free(d.ctx, d.d_multi);
+    d.d_multi = nullptr;
+    d.multi_bytes = 0;
+    void *p = nullptr;
+    if (fap_dev_alloc(d.ctx, bytes, &p) != FAP_OK)
+      return fail(FAP_E_NOMEM, std::string("part blocks: ") + fap_last_error());
+    d.d_multi = (int32_t *)p;
+    d.multi_bytes = bytes;
+  }
+  d.multi_off.clear();
+  std::vector<int32_t> uids;
+  std::vector<int32_t *> blocks;
+  for (auto &kv : off) {
+    uids.push_back(kv.first);
+    blocks.push_back(d.d_multi + kv.second * num_vertexs);
+  }
+  if (batched_) {
+    if (run_batch(uids, blocks) != 0) return -1;
+  } else {
+    for (size_t i = 0; i < uids.size(); i++)
+      if (run(uids[i], blocks[i]) != 0) return -1;
+    if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
+  }
+  d.multi_off = off;
+  current_subgraph_id = ids.empty() ? current_subgraph_id : ids.back();
   if (is_path_needed) error_ = "path matrices are not produced (is_path_needed ignored)";
   return 0;
 }
@@ -401,6 +482,8 @@ const int32_t *fapGraph::subGraphDistanceDevice(int32_t id, int64_t *ld) {
   if (ld) *ld = num_vertexs;
   if (!dev_ || !part_ok(id)) return nullptr;
   if (dev_->block_id == id) return dev_->d_block;
+  auto it = dev_->multi_off.find(id);
+  if (it != dev_->multi_off.end()) return dev_->d_multi + it->second * num_vertexs;
   if (full_solved_) return dev_->d_all + (int64_t)C_BlockVer_offset[id] * num_vertexs;
   return nullptr;
 }

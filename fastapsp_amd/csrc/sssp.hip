@@ -61,8 +61,8 @@ __device__ __forceinline__ void wave_push(bool push, uint32_t item, uint32_t *q,
 template <int NF_THREADS>
 __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
     const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
-    const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld, uint32_t *ws,
-    int64_t ws_stride, int delta) {
+    const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld,
+    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int delta) {
   // Queue counters live in LDS in three sets of {next near, new far}.  Each
   // phase (a near round or a far split) appends into set ci, thread 0 zeroes
   // set ci+1 (mod 3) -- last read before the previous barrier, next written
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
   int round_tag = 0;
   int ci = 0;
   for (int si = blockIdx.x; si < S; si += gridDim.x) {
-    int32_t *dist = out + (int64_t)si * ld;
+    int32_t *dist = rowptr ? rowptr[si] : out + (int64_t)si * ld;
     const int src = sources[si];
     for (int v = tid; v < V; v += NF_THREADS) dist[v] = (v == src) ? 0 : 0x3FFFFFFF;
     uint32_t *qa = st.qa, *qb = st.qb, *fa = st.fa, *fb = st.fb;
@@ -215,8 +215,10 @@ static int pick_delta(const fap_graph_dev *g) {
   return (int)std::max(1.0, std::min(1.0e6, 8.0 * mean));
 }
 
-int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
-             int32_t *d_out, int64_t ld_out, hipStream_t s) {
+// Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
+// pointers) when d_rowptr is not null.
+int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
+                  int32_t *d_out, int64_t ld_out, int32_t *const *d_rowptr, hipStream_t s) {
   if (n_src <= 0) return FAP_OK;
   const int V = g->V;
   if (V <= 0) return fail(FAP_E_ARG, "sssp: empty graph");
@@ -235,19 +237,27 @@ int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int
   switch (ctx->opt_sssp_threads) {
     case 256:
       hipLaunchKernelGGL(fapk::sssp_nf_kernel<256>, dim3(grid), dim3(256), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
+                         delta);
       break;
     case 512:
       hipLaunchKernelGGL(fapk::sssp_nf_kernel<512>, dim3(grid), dim3(512), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
+                         delta);
       break;
     default:
       hipLaunchKernelGGL(fapk::sssp_nf_kernel<1024>, dim3(grid), dim3(1024), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, ws, stride, delta);
+                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
+                         delta);
       break;
   }
   FAP_CHECK_LAUNCH();
   return FAP_OK;
+}
+
+int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
+             int32_t *d_out, int64_t ld_out, hipStream_t s) {
+  return run_sssp_rows(ctx, g, d_sources, n_src, d_out, ld_out, nullptr, s);
 }
 
 }  // namespace fapi

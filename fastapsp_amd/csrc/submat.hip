@@ -75,6 +75,12 @@ namespace fapi {
 
 int run_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
              int32_t *d_out, int64_t ld_out, hipStream_t s);
+int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
+                  int32_t *d_out, int64_t ld_out, int32_t *const *d_rowptr, hipStream_t s);
+int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int32_t *n,
+                   const int64_t *ld, int use_graph, fap_fw_plan **out);
+int fw_plan_run(fap_fw_plan *p, hipStream_t s);
+void fw_plan_destroy(fap_fw_plan *p);
 int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s);
 int launch_minplus(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C,
                    int64_t ldc, int32_t M, int32_t N, int32_t K, int accumulate,
@@ -85,6 +91,15 @@ static int check_part(const fap_graph_dev *g, int32_t start, int32_t n_sub, int3
     return fail(FAP_E_STATE, "graph has no partition (fap_graph_set_partition)");
   if (start < 0 || n_sub < 0 || n_bdy < 0 || n_bdy > n_sub || (int64_t)start + n_sub > g->V)
     return fail(FAP_E_ARG, "part range outside the vertex permutation");
+  // a part is a maximal run of one graph id in st2ed order (fap/fap.cpp:73-78)
+  if (n_sub > 0 && (int64_t)g->h_graph_id.size() == g->V && (int64_t)g->h_st2ed.size() == g->V) {
+    const auto gid = [&](int32_t k) { return g->h_graph_id[g->h_st2ed[k]]; };
+    const int32_t id = gid(start), end = start + n_sub;
+    if (gid(end - 1) != id || (start > 0 && gid(start - 1) == id) ||
+        (end < g->V && gid(end) == id))
+      return fail(FAP_E_ARG, "part range [" + std::to_string(start) + ", " +
+                                 std::to_string(end) + ") is not one whole part");
+  }
   return FAP_OK;
 }
 
@@ -140,11 +155,89 @@ int run_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int3
   return FAP_OK;
 }
 
+// All parts of a batch together: one SSSP launch for every boundary
+// vertex of every part (rows written straight into each part's row block),
+// the part matrices gathered side by side and closed by one batched closure,
+// then the per-part min-plus products and scatters.  Synchronous.
+int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                             const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                             int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
+  if (count <= 0) return FAP_OK;
+  if ((int64_t)g->h_st2ed.size() != g->V)
+    return fail(FAP_E_STATE, "graph has no partition (fap_graph_set_partition)");
+  // stage 1: sources and their output rows
+  std::vector<int32_t> src;
+  std::vector<int32_t *> rowptr;
+  for (int32_t p = 0; p < count; p++)
+    for (int32_t i = 0; i < n_bdys[p]; i++) {
+      src.push_back(g->h_st2ed[starts[p] + i]);
+      rowptr.push_back(rowblocks[p] + (int64_t)i * ld_rb);
+    }
+  const size_t nsrc = src.size();
+  const size_t src_bytes = (nsrc * sizeof(int32_t) + 255) & ~(size_t)255;
+  FAP_TRY(ensure_ws(ctx, ctx->ws_srcs, src_bytes + nsrc * sizeof(int32_t *) + 16));
+  int32_t *d_src = (int32_t *)ctx->ws_srcs.ptr;
+  int32_t **d_rowptr = (int32_t **)((char *)ctx->ws_srcs.ptr + src_bytes);
+  if (nsrc > 0) {
+    FAP_HIP(hipMemcpyAsync(d_src, src.data(), nsrc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    FAP_HIP(hipMemcpyAsync(d_rowptr, rowptr.data(), nsrc * sizeof(int32_t *),
+                           hipMemcpyHostToDevice, s));
+    FAP_TRY(run_sssp_rows(ctx, g, d_src, (int32_t)nsrc, nullptr, 0, d_rowptr, s));
+  }
+  // stage 2: part matrices side by side, one batched closure
+  std::vector<int64_t> ld_sub(count), off(count);
+  std::vector<int32_t *> subs(count);
+  int64_t words = 0;
+  for (int32_t p = 0; p < count; p++) {
+    ld_sub[p] = ((int64_t)n_subs[p] + 3) & ~(int64_t)3;
+    off[p] = words;
+    words += ld_sub[p] * n_subs[p];
+  }
+  FAP_TRY(ensure_ws(ctx, ctx->ws_sub, (size_t)std::max<int64_t>(words, 1) * sizeof(int32_t)));
+  for (int32_t p = 0; p < count; p++) {
+    subs[p] = (int32_t *)ctx->ws_sub.ptr + off[p];
+    FAP_TRY(launch_gather(g, starts[p], n_subs[p], n_bdys[p], rowblocks[p], ld_rb, subs[p],
+                          ld_sub[p], s));
+  }
+  fap_fw_plan *plan = nullptr;
+  FAP_TRY(fw_plan_create(ctx, count, subs.data(), n_subs, ld_sub.data(), 0, &plan));
+  int rc = fw_plan_run(plan, s);
+  // stage 3 and the scatter, per part
+  for (int32_t p = 0; p < count && rc == FAP_OK; p++) {
+    const int32_t b = n_bdys[p];
+    rc = launch_minplus(subs[p] + (int64_t)b * ld_sub[p], ld_sub[p], rowblocks[p], ld_rb,
+                        rowblocks[p] + (int64_t)b * ld_rb, ld_rb, n_subs[p] - b, g->V, b, 0, s);
+    if (rc == FAP_OK)
+      rc = launch_scatter(g, starts[p], n_subs[p], n_subs[p], subs[p], ld_sub[p], rowblocks[p],
+                          ld_rb, s);
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  fw_plan_destroy(plan);
+  if (rc != FAP_OK) return rc;
+  if (e != hipSuccess) return hip_fail(e, "subgraph batch", __FILE__, __LINE__);
+  return FAP_OK;
+}
+
 }  // namespace fapi
 
 using namespace fapi;
 
 extern "C" {
+
+int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                             const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                             int32_t *const *rowblocks, int64_t ld_rb, void *stream) {
+  if (!ctx || !g) return fail(FAP_E_ARG, "null ctx/graph");
+  if (count < 0 || (count > 0 && (!starts || !n_subs || !n_bdys || !rowblocks)))
+    return fail(FAP_E_ARG, "subgraph batch: bad arguments");
+  if (ld_rb < g->V) return fail(FAP_E_ARG, "subgraph batch: ld_rb < V");
+  for (int32_t p = 0; p < count; p++) {
+    FAP_TRY(check_part(g, starts[p], n_subs[p], n_bdys[p]));
+    if (n_subs[p] > 0 && !rowblocks[p]) return fail(FAP_E_ARG, "subgraph batch: null row block");
+  }
+  return run_subgraph_solve_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb,
+                                  pick_stream(ctx, stream));
+}
 
 int fap_gather_submat(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int32_t n_sub,
                       int32_t n_bdy, const int32_t *rowblock, int64_t ld_rb, int32_t *sub,

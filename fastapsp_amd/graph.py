@@ -122,6 +122,14 @@ class Graph:
         check(load().fapg_solve_subgraph(self.handle, int(sub_graph_id), int(is_path_needed)),
               "solveSubGraph")
 
+    def setBatched(self, batched: bool):
+        check(load().fapg_set_batched(self.handle, int(batched)))
+
+    def solveSubGraphs(self, ids, is_path_needed: bool = False):
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        check(load().fapg_solve_subgraphs(self.handle, a.size, _p32(a), int(is_path_needed)),
+              "solveSubGraphs")
+
     def getMapping(self) -> np.ndarray:
         out = np.empty(self.num_vertexs, np.int32)
         check(load().fapg_get_mapping(self.handle, _p32(out), None), "getMapping")

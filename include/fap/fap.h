@@ -70,6 +70,7 @@ class fapGraph {
   bool preconditioned_ = false;
   bool full_solved_ = false;
   bool used_metis_ = false;
+  bool batched_ = true;
   int32_t status_ = 0;
   std::string error_;
 
@@ -77,6 +78,7 @@ class fapGraph {
   void reset_status();
   int32_t ensure_device();
   int32_t run(int32_t subgraph_id, int32_t *d_rowblock);
+  int32_t run_batch(const std::vector<int32_t> &ids, const std::vector<int32_t *> &blocks);
   void init_from_csr(int32_t V, std::vector<int32_t> &&ro, std::vector<int32_t> &&col,
                      std::vector<int32_t> &&w, bool directed_graph);
   bool part_ok(int32_t id) const;
@@ -130,6 +132,14 @@ class fapGraph {
                     int *adj_size, int *row_offset, int *col_val, float *weight, int *graph_id);
 
   // ---- additions ----
+  // Batched (default) or plain part loop for solve() and solveSubGraphs():
+  // batched runs one SSSP launch for all boundary vertices, one batched
+  // closure for all part matrices; plain runs the parts one after another.
+  void setBatched(bool batched) { batched_ = batched; }
+  bool isBatched() const { return batched_; }
+  // Several parts at once (1-based ids); each block stays on the device and
+  // the getters accept any id of the last call.
+  int32_t solveSubGraphs(const std::vector<int32_t> &ids, bool is_path_needed = false);
   int32_t status() const { return status_; }
   const std::string &lastError() const { return error_; }
   bool usedMetis() const { return used_metis_; }

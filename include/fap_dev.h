@@ -178,6 +178,15 @@ int fap_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
                        int32_t n_sub, int32_t n_bdy, int32_t *rowblock,
                        int64_t ld_rb, void *stream);
 
+/* Batched form of fap_subgraph_solve over `count` parts (host arrays of
+ * starts, sizes, boundary counts and device row-block pointers, stride
+ * ld_rb): one SSSP launch for all boundary vertices, one batched closure of
+ * all part matrices, then per-part min-plus and scatter.  Same rows as
+ * fap_subgraph_solve on each part.  Synchronous. */
+int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                             const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                             int32_t *const *rowblocks, int64_t ld_rb, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

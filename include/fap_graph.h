@@ -47,6 +47,10 @@ int fapg_precondition(fapg *g);
 int fapg_is_split(fapg *g, int *out);
 int fapg_solve(fapg *g, int is_path_needed);
 int fapg_solve_subgraph(fapg *g, int32_t id, int is_path_needed);
+/* batched (1, default) or plain (0) part loop for solve / solve_subgraphs */
+int fapg_set_batched(fapg *g, int batched);
+/* several parts at once; each id's rows stay available to the getters */
+int fapg_solve_subgraphs(fapg *g, int32_t count, const int32_t *ids, int is_path_needed);
 
 int fapg_num_vertices(fapg *g, int32_t *V);
 int fapg_num_edges(fapg *g, int64_t *E);

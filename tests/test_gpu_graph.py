@@ -124,3 +124,28 @@ def test_ops_float_mirror():
     sub = np.zeros((5, V), np.float32)
     ops.handle_boundry_path(sub, V, len(col), 5, ro, col, wt.astype(np.float32), st2ed, 10)
     np.testing.assert_array_equal(O.from_ref(sub), O.batched_sssp(st2ed[10:15], ro, col, wt))
+
+
+def test_batched_and_plain_solve_agree():
+    """solve() runs the batched form by default; the plain part loop and the
+    oracle give the same full matrix.  solveSubGraphs keeps several parts'
+    rows available at once."""
+    V, u, v, w, ro, col, wt = grid_csr(48, 0.08, 0.03, 1, 40, 11)
+    want = None
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), 9) as g:
+        g.preCondition()
+        g.solve()
+        D = g.getAllDistanceInt32()
+        want = O.batched_sssp(g.getMapping(), ro, col, wt)
+        np.testing.assert_array_equal(D, want)
+        g.setBatched(False)
+        g.solve()
+        np.testing.assert_array_equal(g.getAllDistanceInt32(), want)
+        vn, vo, _, _ = g.part_info()
+        for batched in (True, False):
+            g.setBatched(batched)
+            g.solveSubGraphs([4, 9, 1, 4])
+            assert g.getCurrentSubGraphId() == 4
+            for pid in (1, 4, 9):
+                np.testing.assert_array_equal(g.getSubGraphDistanceInt32(pid),
+                                              want[vo[pid]:vo[pid] + vn[pid]])

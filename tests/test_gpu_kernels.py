@@ -10,6 +10,7 @@ import pytest
 
 import oracle_lib as O
 from fastapsp_amd import synth
+import fastapsp_amd as fa
 
 pytestmark = pytest.mark.gpu
 INF = O.INF
@@ -265,4 +266,47 @@ def test_subgraph_solve_matches_oracle(gpu_ctx, R, K):
         got = out.get()
         np.testing.assert_array_equal(got, O.run_part(pid, ro, col, wt, gid, pc))
         out.free()
+    dg.close()
+
+
+@pytest.mark.parametrize("R,K,pick", [(20, 4, None), (33, 7, (7, 2, 5)), (16, 1, None),
+                                       (40, 12, (3, 3, 12, 1))])
+def test_subgraph_solve_batch_matches_per_part(gpu_ctx, R, K, pick):
+    """All parts (or a subset, duplicates included) in one batched call: every
+    row block equals the oracle's per-part pipeline, and a guard column past
+    each block's V columns is left alone."""
+    V, ro, col, wt, gid, pc = partitioned_grid(R, K, 3 * R + K)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    ids = list(pick) if pick else list(range(1, K + 1))
+    ld = V + 5
+    outs = [gpu_ctx.to_device(np.full((int(pc["ver_num"][p]), ld), -3, np.int32)) for p in ids]
+    dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in ids],
+                            [int(pc["ver_num"][p]) for p in ids],
+                            [int(pc["bdy_num"][p]) for p in ids],
+                            [o.ptr for o in outs], ld)
+    for p, o in zip(ids, outs):
+        got = o.get().reshape(-1, ld)
+        np.testing.assert_array_equal(got[:, :V], O.run_part(p, ro, col, wt, gid, pc))
+        assert (got[:, V:] == -3).all()
+        o.free()
+    dg.close()
+
+
+def test_subgraph_solve_batch_rejects_bad_parts(gpu_ctx):
+    V, ro, col, wt, gid, pc = partitioned_grid(12, 3, 1)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    out = gpu_ctx.alloc((V, V))
+    s, n, b = int(pc["ver_off"][2]), int(pc["ver_num"][2]), int(pc["bdy_num"][2])
+    with pytest.raises(fa.FapError):
+        dg.subgraph_solve_batch([s], [n], [n + 1], [out.ptr], V)   # n_bdy > n_sub
+    with pytest.raises(fa.FapError):
+        dg.subgraph_solve_batch([s], [n], [b], [out.ptr], V - 1)   # ld < V
+    with pytest.raises(fa.FapError):
+        dg.subgraph_solve_batch([s + 1], [n - 1], [b], [out.ptr], V)   # not a part start
+    with pytest.raises(fa.FapError):
+        dg.subgraph_solve_batch([s], [n + 1], [b], [out.ptr], V)   # runs into the next part
+    dg.subgraph_solve_batch([], [], [], [], V)                     # empty batch is fine
+    out.free()
     dg.close()
