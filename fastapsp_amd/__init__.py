@@ -29,6 +29,29 @@ _pi32 = C.POINTER(C.c_int32)
 _pf32 = C.POINTER(C.c_float)
 _pu64 = C.POINTER(C.c_uint64)
 
+
+
+class ServeOpts(C.Structure):
+    """fap_serve_opts (include/fap_serve.h)."""
+    _fields_ = [("hbm_budget", C.c_int64), ("stage_bytes", C.c_int64), ("lanes", C.c_int32),
+                ("max_batch", C.c_int32), ("max_sources", C.c_int32), ("poll_us", C.c_int32)]
+
+
+class ServeInfo(C.Structure):
+    """fap_serve_info (include/fap_serve.h)."""
+    _fields_ = [("state", C.c_int32), ("kind", C.c_int32), ("n", C.c_int32),
+                ("wave_size", C.c_int32), ("wave", C.c_int64), ("ld", C.c_int64),
+                ("submit_us", C.c_double), ("admit_us", C.c_double), ("done_us", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    """fap_serve_stats (include/fap_serve.h)."""
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "pending", "running",
+                                         "waves", "arena_bytes", "arena_used", "arena_peak",
+                                         "fw_ops", "sssp_sources")] + \
+               [("max_wave", C.c_int32), ("lanes", C.c_int32)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "fap_last_error": (C.c_char_p, []),
@@ -112,6 +135,21 @@ _SIGS = {
     "fapg_distance_p2p": (_int, [_vp, _i32, _i32, _pi32]),
     "fapg_subgraph_checksum": (_int, [_vp, _i32, _pu64, _pu64]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
+    # serving (include/fap_serve.h)
+    "fap_serve_default_opts": (None, [C.POINTER(ServeOpts)]),
+    "fap_serve_create": (_int, [_int, C.POINTER(ServeOpts), C.POINTER(_vp)]),
+    "fap_serve_destroy": (_int, [_vp]),
+    "fap_serve_attach_graph": (_int, [_vp, _i32, _i64, _pi32, _pi32, _pi32]),
+    "fap_serve_submit_graph": (_int, [_vp, _i32, _i64, _pi32, _pi32, _pi32,
+                                      C.POINTER(_i64)]),
+    "fap_serve_submit_queries": (_int, [_vp, _i32, _pi32, _pi32, C.POINTER(_i64)]),
+    "fap_serve_wait": (_int, [_vp, _i64, _i32, C.POINTER(ServeInfo)]),
+    "fap_serve_answers": (_int, [_vp, _i64, _pi32]),
+    "fap_serve_rows": (_int, [_vp, _i64, _i32, _i32, _pi32]),
+    "fap_serve_checksum": (_int, [_vp, _i64, _pu64, _pu64]),
+    "fap_serve_block": (_int, [_vp, _i64, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
+    "fap_serve_release": (_int, [_vp, _i64]),
+    "fap_serve_get_stats": (_int, [_vp, C.POINTER(ServeStats)]),
 }
 
 _lib = None
@@ -368,4 +406,5 @@ class DeviceGraph:
 
 
 from .graph import Graph  # noqa: E402  (the fap::fapGraph drop-in)
+from .serve import Server  # noqa: E402  (request loop)
 from . import ops  # noqa: E402,F401

@@ -10,8 +10,10 @@
 //    rest(r) for r = 0 .. nb-1 -- each phase a set of independent tasks;
 //  * phase p of every job goes into launch p: a queue of tasks, largest
 //    jobs first, consumed by a persistent grid (workgroup w takes tasks w,
-//    w + grid, ...).  Tasks of one launch never depend on each other; the
-//    kernel boundary orders phase p after phase p-1;
+//    w + grid, ...).  The queue is described by one segment per job phase
+//    (its first task and count), found by binary search, so descriptors
+//    grow with the number of phases, not tiles.  Tasks of one launch never
+//    depend on each other; the kernel boundary orders phase p after p-1;
 //  * the launch sequence of a batch is a plan: descriptors uploaded once,
 //    launches captured in a hipGraph and replayed.
 //
@@ -24,21 +26,13 @@
 #include <numeric>
 #include <vector>
 
+#include "batch_desc.h"
 #include "common.h"
 #include "minplus_tile.h"
 
 namespace fapk {
 
-struct BatchJob {
-  int32_t *mat;
-  int64_t ld;
-  int32_t n;
-  int32_t nb;
-};
 
-enum : int { TASK_DIAG = 0, TASK_PANEL = 1, TASK_REST = 2 };
-
-constexpr int BT = 64;  // tile
 constexpr int BT_LD = BT + 4;
 constexpr int BATCH_LDS_WORDS =
     (MpSmall::LDS_WORDS > BT * BT_LD) ? MpSmall::LDS_WORDS : BT * BT_LD;
@@ -76,12 +70,19 @@ __device__ __forceinline__ void batch_diag(int32_t *blk, int64_t ld, int kb, int
 }
 
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_phase_kernel(
-    const BatchJob *__restrict__ jobs, const int4 *__restrict__ tasks, int ntasks) {
+    const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, int nseg, int ntasks) {
   __shared__ __attribute__((aligned(16))) int32_t lds[BATCH_LDS_WORDS];
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
-    const int4 task = tasks[ti];
-    const BatchJob job = jobs[task.x];
-    const int kind = task.y & 3, r = task.y >> 2, idx = task.z;
+    // the segment holding task ti: segments tile [0, ntasks) in order
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].first <= ti) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchSeg sg = segs[lo];
+    const BatchJob job = jobs[sg.job];
+    const int kind = sg.kind_r & 3, r = sg.kind_r >> 2, idx = ti - sg.first;
     const int nb = job.nb, n = job.n;
     const int64_t ld = job.ld;
     int32_t *d = job.mat;
@@ -120,23 +121,28 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 
 namespace fapi {
 
-// Tasks of a batch grouped by phase: group p holds phase p of every job that
-// has one, jobs in decreasing size (longest processing time first).  A task
-// is {job, kind | r << 2, index, 0}; a one-block job has only its diagonal.
-void build_fw_phases(const std::vector<int32_t> &n, std::vector<int4> *tasks,
-                     std::vector<int32_t> *group_off) {
-  const int J = (int)n.size();
-  std::vector<int32_t> order(J);
+// Launch p of a batch holds phase p of every job that has one, jobs in
+// decreasing size (longest processing time first), each job's phase one
+// segment of consecutive tasks.  A one-block job has only its diagonal.
+void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
+                    FwBatchDesc *out) {
+  out->jobs.resize(count);
+  out->segs.clear();
+  out->seg_off.assign(1, 0);
+  out->launch_tasks.clear();
+  out->ops = 0;
+  std::vector<int32_t> order(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
   int max_phases = 0;
-  for (int j = 0; j < J; j++) {
+  for (int j = 0; j < count; j++) {
     const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
+    out->jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], nb};
+    out->ops += 2 * (int64_t)n[j] * n[j] * n[j];
     max_phases = std::max(max_phases, nb > 1 ? 3 * nb : (n[j] > 0 ? 1 : 0));
   }
-  tasks->clear();
-  group_off->assign(1, 0);
   for (int p = 0; p < max_phases; p++) {
+    int32_t first = 0;
     for (int j : order) {
       if (n[j] <= 0) continue;
       const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
@@ -144,24 +150,38 @@ void build_fw_phases(const std::vector<int32_t> &n, std::vector<int4> *tasks,
       if (p >= phases) continue;
       const int r = nb > 1 ? p / 3 : 0;
       const int kind = nb > 1 ? p % 3 : fapk::TASK_DIAG;
-      const int count = kind == fapk::TASK_DIAG    ? 1
-                        : kind == fapk::TASK_PANEL ? 2 * (nb - 1)
-                                                   : (nb - 1) * (nb - 1);
-      for (int i = 0; i < count; i++) tasks->push_back(make_int4(j, kind | (r << 2), i, 0));
+      const int cnt = kind == fapk::TASK_DIAG    ? 1
+                      : kind == fapk::TASK_PANEL ? 2 * (nb - 1)
+                                                 : (nb - 1) * (nb - 1);
+      out->segs.push_back(fapk::BatchSeg{j, kind | (r << 2), first, cnt});
+      first += cnt;
     }
-    group_off->push_back((int32_t)tasks->size());
+    out->launch_tasks.push_back(first);
+    out->seg_off.push_back((int32_t)out->segs.size());
   }
+}
+
+int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
+                    const fapk::BatchSeg *d_segs, hipStream_t s) {
+  const int cap = ctx->num_cus * 4;
+  for (size_t p = 0; p < desc.launch_tasks.size(); p++) {
+    const int cnt = desc.launch_tasks[p];
+    const int nseg = desc.seg_off[p + 1] - desc.seg_off[p];
+    if (cnt <= 0 || nseg <= 0) continue;
+    hipLaunchKernelGGL(fapk::fw_phase_kernel, dim3(std::min(cnt, cap)), dim3(fapk::MP_THREADS),
+                       0, s, d_jobs, d_segs + desc.seg_off[p], nseg, cnt);
+    FAP_CHECK_LAUNCH();
+  }
+  return FAP_OK;
 }
 
 }  // namespace fapi
 
 struct fap_fw_plan {
   fap_ctx *ctx = nullptr;
-  void *desc = nullptr;        // jobs | tasks (device)
-  int njobs = 0;
-  size_t tasks_off = 0;
-  std::vector<int32_t> group_off;
-  int64_t ops = 0;
+  void *desc = nullptr;  // jobs | segments (device)
+  size_t segs_off = 0;
+  fapi::FwBatchDesc host;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -169,42 +189,22 @@ struct fap_fw_plan {
 namespace fapi {
 
 static int plan_launch(fap_fw_plan *p, hipStream_t s) {
-  const fapk::BatchJob *jobs = (const fapk::BatchJob *)p->desc;
-  const int4 *tasks = (const int4 *)((char *)p->desc + p->tasks_off);
-  const int cap = p->ctx->num_cus * 4;
-  for (size_t g = 0; g + 1 < p->group_off.size(); g++) {
-    const int cnt = p->group_off[g + 1] - p->group_off[g];
-    if (cnt <= 0) continue;
-    hipLaunchKernelGGL(fapk::fw_phase_kernel, dim3(std::min(cnt, cap)), dim3(fapk::MP_THREADS),
-                       0, s, jobs, tasks + p->group_off[g], cnt);
-    FAP_CHECK_LAUNCH();
-  }
-  return FAP_OK;
+  return launch_fw_batch(p->ctx, p->host, (const fapk::BatchJob *)p->desc,
+                         (const fapk::BatchSeg *)((char *)p->desc + p->segs_off), s);
 }
 
 int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int32_t *n,
                    const int64_t *ld, int use_graph, fap_fw_plan **out) {
-  std::vector<int32_t> sizes(n, n + count);
-  std::vector<int4> tasks;
-  std::vector<int32_t> goff;
-  build_fw_phases(sizes, &tasks, &goff);
-  std::vector<fapk::BatchJob> jobs(count);
-  int64_t ops = 0;
-  for (int j = 0; j < count; j++) {
-    jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], (n[j] + fapk::BT - 1) / fapk::BT};
-    ops += 2 * (int64_t)n[j] * n[j] * n[j];
-  }
   fap_fw_plan *p = new fap_fw_plan();
   p->ctx = ctx;
-  p->njobs = count;
-  p->group_off = goff;
-  p->ops = ops;
+  build_fw_batch(count, mats, n, ld, &p->host);
   const size_t jobs_b = sizeof(fapk::BatchJob) * count;
-  p->tasks_off = (jobs_b + 255) & ~(size_t)255;
-  const size_t total = p->tasks_off + sizeof(int4) * std::max<size_t>(tasks.size(), 1);
+  p->segs_off = (jobs_b + 255) & ~(size_t)255;
+  const size_t segs_b = sizeof(fapk::BatchSeg) * p->host.segs.size();
+  const size_t total = p->segs_off + std::max<size_t>(segs_b, 16);
   std::vector<char> host(total, 0);
-  if (jobs_b) memcpy(host.data(), jobs.data(), jobs_b);
-  if (!tasks.empty()) memcpy(host.data() + p->tasks_off, tasks.data(), sizeof(int4) * tasks.size());
+  if (jobs_b) memcpy(host.data(), p->host.jobs.data(), jobs_b);
+  if (segs_b) memcpy(host.data() + p->segs_off, p->host.segs.data(), segs_b);
   hipError_t e = hipMalloc(&p->desc, total);
   if (e == hipSuccess) e = hipMemcpy(p->desc, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -212,7 +212,7 @@ int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int3
     delete p;
     return hip_fail(e, "batch plan descriptors", __FILE__, __LINE__);
   }
-  if (use_graph && !tasks.empty()) {
+  if (use_graph && !p->host.segs.empty()) {
     hipStream_t cs = ctx->aux_stream;  // capture on a stream with no other work queued
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     int rc = FAP_OK;
@@ -301,9 +301,11 @@ int fap_fw_plan_run(fap_fw_plan *plan, void *stream) {
 
 int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks, int64_t *ops) {
   if (!plan) return fail(FAP_E_ARG, "null plan");
-  if (launches) *launches = (int32_t)plan->group_off.size() - 1;
-  if (tasks) *tasks = plan->group_off.empty() ? 0 : plan->group_off.back();
-  if (ops) *ops = plan->ops;
+  int64_t t = 0;
+  for (int32_t c : plan->host.launch_tasks) t += c;
+  if (launches) *launches = (int32_t)plan->host.launch_tasks.size();
+  if (tasks) *tasks = t;
+  if (ops) *ops = plan->host.ops;
   return FAP_OK;
 }
 

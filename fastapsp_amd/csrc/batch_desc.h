@@ -1,0 +1,80 @@
+// batch_desc.h -- descriptors shared by the batched closure (batch.hip),
+// the dense matrix build (dense.hip) and the request loop (serve.hip).
+//
+// A batch is described by plain structs that the host writes once (into
+// pinned or pageable memory) and copies to the device in one transfer; the
+// launch functions take the device copies, so a caller that owns its own
+// staging (the request loop) never waits on the host between launches.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.h"
+
+namespace fapk {
+
+constexpr int BT = 64;  // batched closure tile
+
+// One matrix of a batched closure.
+struct BatchJob {
+  int32_t *mat;
+  int64_t ld;
+  int32_t n;
+  int32_t nb;  // ceil(n / BT)
+};
+
+// One phase of one job: tasks [first, first + count) of its launch are the
+// tiles of that phase.  kind_r = kind | (pivot block << 2).
+struct BatchSeg {
+  int32_t job;
+  int32_t kind_r;
+  int32_t first;
+  int32_t count;
+};
+
+enum : int { TASK_DIAG = 0, TASK_PANEL = 1, TASK_REST = 2 };
+
+// One graph's dense matrix: INF, arcs in CSR order, zero diagonal.
+struct DenseJob {
+  const int32_t *row_offset;
+  const int2 *edges;
+  int32_t *mat;
+  int64_t ld;
+  int64_t cell_base;  // prefix of n^2 over earlier jobs
+  int32_t row_base;   // prefix of n over earlier jobs
+  int32_t n;
+};
+
+}  // namespace fapk
+
+namespace fapi {
+
+// Host side of a batched closure: jobs, the segments of every launch (launch
+// p = segments [seg_off[p], seg_off[p + 1])) and the task count per launch.
+struct FwBatchDesc {
+  std::vector<fapk::BatchJob> jobs;
+  std::vector<fapk::BatchSeg> segs;
+  std::vector<int32_t> seg_off;
+  std::vector<int32_t> launch_tasks;
+  int64_t ops = 0;
+};
+
+void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
+                    FwBatchDesc *out);
+// Launch sequence of a batch whose jobs and segments are already on the
+// device (same order as in `desc`).
+int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
+                    const fapk::BatchSeg *d_segs, hipStream_t s);
+
+// Multi-source SSSP (sssp.hip): source i's distances go to d_out + i * ld_out,
+// or to d_rowptr[i] when d_rowptr is given (a device array of row pointers).
+int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
+                  int32_t *d_out, int64_t ld_out, int32_t *const *d_rowptr, hipStream_t s);
+
+// Dense build of jobs already on the device; `cells` = sum n^2, `rows` = sum n.
+int launch_dense_jobs(fap_ctx *ctx, const fapk::DenseJob *d_jobs, int32_t njobs, int64_t cells,
+                      int64_t rows, hipStream_t s);
+
+}  // namespace fapi

@@ -88,17 +88,17 @@ struct fap_ctx {
   int opt_fw_lookahead = 0;      // run round r+1's phases 1-2 beside round r's phase 3
   int opt_fw_tile = 0;           // FW block size 64 / 128; 0 = by n
   int opt_sssp_threads = 1024;   // threads per SSSP workgroup: 256, 512, 1024
-  int opt_sssp_delta = 0;        // near-far band; 0 = per graph (4 x mean weight)
+  int opt_sssp_delta = 0;        // near-far band; 0 = per graph (8 x mean weight)
 };
 
 namespace fapi {
 // Ensure ws holds >= bytes (device memory of ctx's device). Synchronizes the
-// context stream before reallocating.
+// device before reallocating.
 int ensure_ws(fap_ctx *ctx, Workspace &ws, size_t bytes);
 inline hipStream_t pick_stream(fap_ctx *ctx, void *stream) {
   return stream ? (hipStream_t)stream : ctx->stream;
 }
-// Device-side helpers implemented in util.hip.
+// Device-side helpers implemented in ctx.hip.
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);
 }  // namespace fapi

@@ -9,19 +9,10 @@
 #include <algorithm>
 #include <vector>
 
+#include "batch_desc.h"
 #include "common.h"
 
 namespace fapk {
-
-struct DenseJob {
-  const int32_t *row_offset;
-  const int2 *edges;
-  int32_t *mat;
-  int64_t ld;
-  int64_t cell_base;  // prefix of n^2 over earlier jobs
-  int32_t row_base;   // prefix of n over earlier jobs
-  int32_t n;
-};
 
 __device__ __forceinline__ int find_job(const DenseJob *jobs, int count, int64_t x, bool rows) {
   int lo = 0, hi = count - 1;
@@ -61,6 +52,20 @@ __global__ void dense_arcs_kernel(const DenseJob *__restrict__ jobs, int count, 
 
 namespace fapi {
 
+int launch_dense_jobs(fap_ctx *ctx, const fapk::DenseJob *d_jobs, int32_t njobs, int64_t cells,
+                      int64_t rows, hipStream_t s) {
+  if (njobs <= 0 || cells <= 0) return FAP_OK;
+  if (rows > INT32_MAX) return fail(FAP_E_ARG, "dense build: too many rows");
+  const int64_t blocks = std::min<int64_t>((cells + 255) / 256, (int64_t)ctx->num_cus * 32);
+  hipLaunchKernelGGL(fapk::dense_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs,
+                     njobs, cells);
+  FAP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(fapk::dense_arcs_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
+                     s, d_jobs, njobs, (int32_t)rows);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
 int run_dense_from_graphs(fap_ctx *ctx, int32_t count, const fap_graph_dev *const *graphs,
                           int32_t *const *mats, const int64_t *ld, hipStream_t s) {
   std::vector<fapk::DenseJob> jobs;
@@ -80,14 +85,8 @@ int run_dense_from_graphs(fap_ctx *ctx, int32_t count, const fap_graph_dev *cons
   const size_t bytes = sizeof(fapk::DenseJob) * jobs.size();
   FAP_TRY(ensure_ws(ctx, ctx->ws_dense, bytes));
   FAP_HIP(hipMemcpyAsync(ctx->ws_dense.ptr, jobs.data(), bytes, hipMemcpyHostToDevice, s));
-  const fapk::DenseJob *dj = (const fapk::DenseJob *)ctx->ws_dense.ptr;
-  const int64_t blocks = std::min<int64_t>((cells + 255) / 256, (int64_t)ctx->num_cus * 32);
-  hipLaunchKernelGGL(fapk::dense_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dj,
-                     (int)jobs.size(), cells);
-  FAP_CHECK_LAUNCH();
-  hipLaunchKernelGGL(fapk::dense_arcs_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
-                     s, dj, (int)jobs.size(), (int32_t)rows);
-  FAP_CHECK_LAUNCH();
+  FAP_TRY(launch_dense_jobs(ctx, (const fapk::DenseJob *)ctx->ws_dense.ptr, (int32_t)jobs.size(),
+                            cells, rows, s));
   // the descriptor copy reads `jobs` asynchronously
   FAP_HIP(hipStreamSynchronize(s));
   return FAP_OK;

@@ -1,0 +1,825 @@
+// serve.hip -- request loop with continuous batching (include/fap_serve.h).
+//
+// No counterpart in the reference: it replaces the multi-task MPI driver
+// (fap/fap_more_task_mpi.cpp:34-91) on one device and answers point
+// queries (getDistanceP2P, a stub at fap/fap.cpp:263-265).
+//
+// One host thread runs the loop:
+//   1. completions: a lane's wave is done when its event is; query answers
+//      are copied out of the lane's pinned staging and the query rows freed;
+//   2. admissions: every free lane takes a wave from the queue -- requests
+//      of the head's kind, in arrival order, while the arena, the staging
+//      and the wave limits allow -- stages it and launches it;
+//   3. waiting: on the queue when no lane is busy, else a short poll.
+// Device memory for results comes from one arena (first fit, coalescing),
+// so admission is by footprint and no hipMalloc/hipFree runs per request.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <list>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_desc.h"
+#include "common.h"
+#include "fap_serve.h"
+
+namespace fapk {
+
+// ans[q] = row of pair q's source, at pair q's destination column
+__global__ void answers_kernel(int32_t *const *__restrict__ rowptr,
+                               const int32_t *__restrict__ qrow,
+                               const int32_t *__restrict__ qdst, int32_t *__restrict__ ans,
+                               int nq) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) ans[q] = rowptr[qrow[q]][qdst[q]];
+}
+
+// fap_checksum_i32 of an n x n matrix stored with row stride ld, indexed as
+// if packed (cell (i, j) has position i * n + j).
+__global__ void checksum2d_kernel(const uint32_t *__restrict__ d, int64_t ld, int32_t n,
+                                  unsigned long long *out) {
+  unsigned long long s = 0, ws = 0;
+  const int64_t cells = (int64_t)n * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < cells; x += stride) {
+    const int64_t i = x / n, j = x - (x / n) * n;
+    const unsigned long long v = d[i * ld + j];
+    s += v;
+    ws += v * (unsigned long long)(x + 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off);
+    ws += __shfl_down(ws, off);
+  }
+  __shared__ unsigned long long ps[4], pw[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    ps[wv] = s;
+    pw[wv] = ws;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(out, ps[0] + ps[1] + ps[2] + ps[3]);
+    atomicAdd(out + 1, pw[0] + pw[1] + pw[2] + pw[3]);
+  }
+}
+
+}  // namespace fapk
+
+namespace fapi {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over one device allocation.  Offsets are 256-byte
+// aligned; freed blocks merge with free neighbours.
+class Arena {
+ public:
+  void init(size_t size) {
+    size_ = size;
+    used_ = peak_ = 0;
+    free_.clear();
+    live_.clear();
+    if (size) free_[0] = size;
+  }
+  int64_t alloc(size_t bytes) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, sz = it->second;
+      free_.erase(it);
+      if (sz > bytes) free_[off + bytes] = sz - bytes;
+      live_[off] = bytes;
+      used_ += bytes;
+      peak_ = std::max(peak_, used_);
+      return (int64_t)off;
+    }
+    return -1;
+  }
+  void release(int64_t off) {
+    auto it = live_.find((size_t)off);
+    if (it == live_.end()) return;
+    size_t start = it->first, len = it->second;
+    used_ -= len;
+    live_.erase(it);
+    auto nx = free_.lower_bound(start);
+    if (nx != free_.end() && nx->first == start + len) {
+      len += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == start) {
+        start = pv->first;
+        len += pv->second;
+        free_.erase(pv);
+      }
+    }
+    free_[start] = len;
+  }
+  size_t size() const { return size_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  size_t size_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  std::map<size_t, size_t> live_;
+};
+
+struct Request {
+  int64_t id = 0;
+  int kind = FAP_REQ_GRAPH;
+  int state = FAP_REQ_PENDING;
+  int code = FAP_OK;
+  std::string error;
+  // graph
+  int32_t V = 0;
+  int64_t E = 0;
+  std::vector<int32_t> row_offset;
+  std::vector<int2> edges;
+  int64_t block = -1;  // arena offset of the n x ld result
+  int64_t ld = 0;
+  // queries
+  std::vector<int32_t> dst, uniq, qrow, answers;
+  int64_t rows = -1;  // arena offset of uniq.size() x V distance rows
+  // bookkeeping
+  int64_t wave = -1;
+  int32_t wave_size = 0;
+  int readers = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+
+  size_t stage_need() const {
+    if (kind == FAP_REQ_GRAPH)
+      return align_up(4 * ((size_t)V + 1)) + align_up(8 * (size_t)E) + sizeof(fapk::DenseJob) +
+             sizeof(fapk::BatchJob) + 3 * sizeof(fapk::BatchSeg) * (size_t)((V + 63) / 64 + 1);
+    const size_t q = dst.size(), u = uniq.size();
+    return 4 * u + 8 * u + 12 * q;
+  }
+  size_t arena_need(int32_t resident_V) const {
+    if (kind == FAP_REQ_GRAPH) return 4 * (size_t)V * (size_t)ld;
+    return 4 * uniq.size() * (size_t)resident_V;
+  }
+};
+
+// Slack for the alignment of a wave's descriptor arrays in staging.
+constexpr size_t kStageSlack = 8 * kAlign;
+
+struct Lane {
+  fap_ctx *ctx = nullptr;
+  char *h_stage = nullptr;  // pinned host
+  char *d_stage = nullptr;  // device
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  int kind = FAP_REQ_GRAPH;
+  std::vector<Request *> wave;
+  size_t ans_off = 0;
+};
+
+}  // namespace
+}  // namespace fapi
+
+struct fap_server {
+  int device = 0;
+  fap_serve_opts opts{};
+  std::vector<fapi::Lane> lanes;
+  char *arena_base = nullptr;
+  fapi::Arena arena;
+  fap_graph_dev *resident = nullptr;
+  // client-side device work (rows, checksums) runs on its own stream
+  hipStream_t client = nullptr;
+  unsigned long long *d_acc = nullptr;
+  std::mutex client_mu;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::list<fapi::Request *> pending;
+  std::unordered_map<int64_t, fapi::Request *> reqs;
+  int64_t next_id = 1, next_wave = 0;
+  uint64_t kick = 0;  // bumped by submit / release / stop
+  bool stop = false;
+  std::string fatal;
+  std::thread thr;
+  fapi::Clock::time_point t0;
+  fap_serve_stats st{};
+  double now_us() const {
+    return std::chrono::duration<double, std::micro>(fapi::Clock::now() - t0).count();
+  }
+};
+
+namespace fapi {
+namespace {
+
+void finish(fap_server *S, Request *r, int state, int code, const std::string &err) {
+  r->state = state;
+  r->code = code;
+  r->error = err;
+  r->t_done = S->now_us();
+  if (state == FAP_REQ_FAILED) {
+    S->st.failed++;
+    if (r->block >= 0) S->arena.release(r->block);
+    r->block = -1;
+  } else {
+    S->st.completed++;
+  }
+  if (r->rows >= 0) S->arena.release(r->rows);
+  r->rows = -1;
+  // the inputs were staged; free them
+  std::vector<int32_t>().swap(r->row_offset);
+  std::vector<int2>().swap(r->edges);
+}
+
+// Picks the next wave for lane L (lock held).  Returns true if anything
+// changed (requests admitted or failed).
+bool select_wave(fap_server *S, Lane &L) {
+  if (S->pending.empty()) return false;
+  const int kind = S->pending.front()->kind;
+  const size_t stage_cap = (size_t)S->opts.stage_bytes - kStageSlack;
+  const int32_t rV = S->resident ? S->resident->V : 0;
+  size_t stage = 0;
+  int64_t sources = 0;
+  bool changed = false;
+  L.wave.clear();
+  for (auto it = S->pending.begin(); it != S->pending.end();) {
+    if ((int)L.wave.size() >= S->opts.max_batch) break;
+    Request *r = *it;
+    if (r->kind != kind) {
+      ++it;
+      continue;
+    }
+    if (kind == FAP_REQ_QUERY && !S->resident) {
+      finish(S, r, FAP_REQ_FAILED, FAP_E_STATE, "no resident graph attached");
+      it = S->pending.erase(it);
+      changed = true;
+      continue;
+    }
+    const size_t sn = r->stage_need();
+    const int64_t un = kind == FAP_REQ_QUERY ? (int64_t)r->uniq.size() : 0;
+    if (sn > stage_cap || un > S->opts.max_sources) {
+      finish(S, r, FAP_REQ_FAILED, FAP_E_NOMEM,
+             "request larger than one wave (staging bytes or sources per wave)");
+      it = S->pending.erase(it);
+      changed = true;
+      continue;
+    }
+    if (stage + sn > stage_cap || sources + un > S->opts.max_sources) break;
+    const size_t an = r->arena_need(rV);
+    if (an > S->arena.size()) {
+      finish(S, r, FAP_REQ_FAILED, FAP_E_NOMEM, "request larger than the HBM budget");
+      it = S->pending.erase(it);
+      changed = true;
+      continue;
+    }
+    const int64_t off = S->arena.alloc(an);
+    if (off < 0) break;  // wait for memory; later requests of this kind wait too
+    if (kind == FAP_REQ_GRAPH) r->block = off;
+    else r->rows = off;
+    stage += sn;
+    sources += un;
+    r->state = FAP_REQ_RUNNING;
+    r->t_admit = S->now_us();
+    L.wave.push_back(r);
+    it = S->pending.erase(it);
+    changed = true;
+  }
+  L.kind = kind;
+  if (!L.wave.empty()) {
+    const int64_t w = S->next_wave++;
+    for (Request *r : L.wave) {
+      r->wave = w;
+      r->wave_size = (int32_t)L.wave.size();
+    }
+  }
+  return changed;
+}
+
+// Stages and launches lane L's wave (lock not held; the wave's requests are
+// RUNNING, so no client call touches them).
+int launch_wave(fap_server *S, Lane &L) {
+  char *h = L.h_stage;
+  size_t off = 0;
+  auto put = [&](const void *src, size_t bytes) {
+    const size_t o = off;
+    if (bytes) memcpy(h + o, src, bytes);
+    off = align_up(o + bytes);
+    return o;
+  };
+  hipStream_t s = L.ctx->stream;
+  FAP_HIP(hipSetDevice(S->device));
+  if (L.kind == FAP_REQ_GRAPH) {
+    const int32_t cnt = (int32_t)L.wave.size();
+    std::vector<size_t> o_ro(cnt), o_e(cnt);
+    std::vector<int32_t *> mats(cnt);
+    std::vector<int32_t> ns(cnt);
+    std::vector<int64_t> lds(cnt);
+    for (int32_t j = 0; j < cnt; j++) {
+      Request *r = L.wave[j];
+      o_ro[j] = put(r->row_offset.data(), 4 * ((size_t)r->V + 1));
+      o_e[j] = put(r->edges.data(), 8 * (size_t)r->E);
+      mats[j] = (int32_t *)(S->arena_base + r->block);
+      ns[j] = r->V;
+      lds[j] = r->ld;
+    }
+    std::vector<fapk::DenseJob> dj;
+    int64_t cells = 0, rows = 0;
+    for (int32_t j = 0; j < cnt; j++) {
+      dj.push_back(fapk::DenseJob{(const int32_t *)(L.d_stage + o_ro[j]),
+                                  (const int2 *)(L.d_stage + o_e[j]), mats[j], lds[j], cells,
+                                  (int32_t)rows, ns[j]});
+      cells += (int64_t)ns[j] * ns[j];
+      rows += ns[j];
+    }
+    FwBatchDesc fd;
+    build_fw_batch(cnt, mats.data(), ns.data(), lds.data(), &fd);
+    const size_t o_dj = put(dj.data(), sizeof(fapk::DenseJob) * dj.size());
+    const size_t o_bj = put(fd.jobs.data(), sizeof(fapk::BatchJob) * fd.jobs.size());
+    const size_t o_sg = put(fd.segs.data(), sizeof(fapk::BatchSeg) * fd.segs.size());
+    if (off > (size_t)S->opts.stage_bytes) return fail(FAP_E_NOMEM, "wave staging overflow");
+    FAP_HIP(hipMemcpyAsync(L.d_stage, h, off, hipMemcpyHostToDevice, s));
+    FAP_TRY(launch_dense_jobs(L.ctx, (const fapk::DenseJob *)(L.d_stage + o_dj), cnt, cells, rows,
+                              s));
+    FAP_TRY(launch_fw_batch(L.ctx, fd, (const fapk::BatchJob *)(L.d_stage + o_bj),
+                            (const fapk::BatchSeg *)(L.d_stage + o_sg), s));
+  } else {
+    const int32_t V = S->resident->V;
+    std::vector<int32_t> src, qrow, qdst;
+    std::vector<int32_t *> rowptr;
+    for (Request *r : L.wave) {
+      const int32_t base = (int32_t)src.size();
+      int32_t *rows = (int32_t *)(S->arena_base + r->rows);
+      for (size_t k = 0; k < r->uniq.size(); k++) {
+        src.push_back(r->uniq[k]);
+        rowptr.push_back(rows + (int64_t)k * V);
+      }
+      for (size_t q = 0; q < r->dst.size(); q++) {
+        qrow.push_back(base + r->qrow[q]);
+        qdst.push_back(r->dst[q]);
+      }
+    }
+    const size_t o_src = put(src.data(), 4 * src.size());
+    const size_t o_rp = put(rowptr.data(), 8 * rowptr.size());
+    const size_t o_qr = put(qrow.data(), 4 * qrow.size());
+    const size_t o_qd = put(qdst.data(), 4 * qdst.size());
+    const size_t staged = off;
+    L.ans_off = off;
+    off = align_up(off + 4 * qdst.size());
+    if (off > (size_t)S->opts.stage_bytes) return fail(FAP_E_NOMEM, "wave staging overflow");
+    FAP_HIP(hipMemcpyAsync(L.d_stage, h, staged, hipMemcpyHostToDevice, s));
+    FAP_TRY(run_sssp_rows(L.ctx, S->resident, (const int32_t *)(L.d_stage + o_src),
+                          (int32_t)src.size(), nullptr, 0, (int32_t *const *)(L.d_stage + o_rp),
+                          s));
+    const int nq = (int)qdst.size();
+    hipLaunchKernelGGL(fapk::answers_kernel, dim3((nq + 255) / 256), dim3(256), 0, s,
+                       (int32_t *const *)(L.d_stage + o_rp), (const int32_t *)(L.d_stage + o_qr),
+                       (const int32_t *)(L.d_stage + o_qd), (int32_t *)(L.d_stage + L.ans_off),
+                       nq);
+    FAP_CHECK_LAUNCH();
+    FAP_HIP(hipMemcpyAsync(h + L.ans_off, L.d_stage + L.ans_off, 4 * (size_t)nq,
+                           hipMemcpyDeviceToHost, s));
+  }
+  FAP_HIP(hipEventRecord(L.done, s));
+  return FAP_OK;
+}
+
+void complete_wave(fap_server *S, Lane &L) {
+  const int32_t *ans = (const int32_t *)(L.h_stage + L.ans_off);
+  for (Request *r : L.wave) {
+    if (r->kind == FAP_REQ_QUERY) {
+      r->answers.assign(ans, ans + r->dst.size());
+      ans += r->dst.size();
+      S->st.sssp_sources += (int64_t)r->uniq.size();
+    } else {
+      S->st.fw_ops += 2 * (int64_t)r->V * r->V * r->V;
+    }
+    finish(S, r, FAP_REQ_DONE, FAP_OK, "");
+  }
+  L.wave.clear();
+  L.busy = false;
+}
+
+void fail_wave(fap_server *S, Lane &L, int code, const std::string &err) {
+  for (Request *r : L.wave) finish(S, r, FAP_REQ_FAILED, code, err);
+  L.wave.clear();
+  L.busy = false;
+}
+
+void serve_loop(fap_server *S) {
+  (void)hipSetDevice(S->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  uint64_t seen = S->kick;
+  for (;;) {
+    bool progress = false, busy = false;
+    for (Lane &L : S->lanes) {
+      if (!L.busy) continue;
+      const hipError_t e = hipEventQuery(L.done);
+      if (e == hipErrorNotReady) {
+        busy = true;
+        continue;
+      }
+      if (e == hipSuccess) {
+        complete_wave(S, L);
+      } else {
+        (void)hipGetLastError();
+        S->fatal = std::string("device error in a wave: ") + hipGetErrorString(e);
+        fail_wave(S, L, FAP_E_HIP, S->fatal);
+      }
+      progress = true;
+    }
+    if (S->stop && !busy) break;
+    if (!S->stop && S->fatal.empty()) {
+      for (Lane &L : S->lanes) {
+        if (L.busy || S->pending.empty()) continue;
+        if (select_wave(S, L)) progress = true;
+        if (L.wave.empty()) continue;
+        L.busy = true;
+        S->st.waves++;
+        S->st.max_wave = std::max<int32_t>(S->st.max_wave, (int32_t)L.wave.size());
+        lk.unlock();
+        const int rc = launch_wave(S, L);
+        const std::string err = rc == FAP_OK ? std::string() : std::string(fap_last_error());
+        if (rc != FAP_OK) (void)hipStreamSynchronize(L.ctx->stream);
+        lk.lock();
+        if (rc != FAP_OK) fail_wave(S, L, rc, err);
+        busy = true;
+      }
+    }
+    if (progress) {
+      S->cv_done.notify_all();
+      continue;
+    }
+    if (busy) {
+      S->cv_work.wait_for(lk, std::chrono::microseconds(S->opts.poll_us),
+                          [&] { return S->kick != seen; });
+    } else {
+      S->cv_work.wait(lk, [&] { return S->kick != seen; });
+    }
+    seen = S->kick;
+  }
+  for (Request *r : S->pending) finish(S, r, FAP_REQ_FAILED, FAP_E_STATE, "server stopped");
+  S->pending.clear();
+  S->cv_done.notify_all();
+}
+
+int check_csr(int32_t V, int64_t E, const int32_t *ro, const int32_t *col, const int32_t *w) {
+  if (V <= 0 || E < 0 || E > INT32_MAX) return fail(FAP_E_ARG, "bad V/E");
+  if (!ro || (E > 0 && (!col || !w))) return fail(FAP_E_ARG, "null CSR array");
+  if (ro[0] != 0 || ro[V] != E) return fail(FAP_E_ARG, "row_offset must start at 0 and end at E");
+  for (int32_t v = 0; v < V; v++)
+    if (ro[v + 1] < ro[v]) return fail(FAP_E_ARG, "row_offset not monotone");
+  for (int64_t e = 0; e < E; e++) {
+    if (col[e] < 0 || col[e] >= V) return fail(FAP_E_ARG, "col out of range");
+    if (w[e] < 0 || w[e] >= FAP_INF) return fail(FAP_E_ARG, "weight outside [0, INF)");
+  }
+  return FAP_OK;
+}
+
+int64_t enqueue(fap_server *S, Request *r) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->id = S->next_id++;
+  r->t_submit = S->now_us();
+  S->reqs[r->id] = r;
+  S->pending.push_back(r);
+  S->st.submitted++;
+  S->kick++;
+  S->cv_work.notify_all();
+  return r->id;
+}
+
+void fill_info(const Request *r, fap_serve_info *info) {
+  if (!info) return;
+  info->state = r->state;
+  info->kind = r->kind;
+  info->n = r->kind == FAP_REQ_GRAPH ? r->V : (int32_t)r->dst.size();
+  info->wave_size = r->wave_size;
+  info->wave = r->wave;
+  info->ld = r->ld;
+  info->submit_us = r->t_submit;
+  info->admit_us = r->t_admit;
+  info->done_us = r->t_done;
+}
+
+// Looks up a finished graph request and pins it against release.
+int pin_graph(fap_server *S, int64_t id, Request **out) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(FAP_E_ARG, "unknown request id");
+  Request *r = it->second;
+  if (r->kind != FAP_REQ_GRAPH) return fail(FAP_E_ARG, "not a graph request");
+  if (r->state != FAP_REQ_DONE) return fail(FAP_E_BUSY, "request not finished");
+  r->readers++;
+  *out = r;
+  return FAP_OK;
+}
+
+void unpin(fap_server *S, Request *r) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->readers--;
+}
+
+void free_server(fap_server *S) {
+  for (Lane &L : S->lanes) {
+    if (L.ctx) (void)hipStreamSynchronize(L.ctx->stream);
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.h_stage) (void)hipHostFree(L.h_stage);
+    if (L.d_stage) (void)hipFree(L.d_stage);
+    if (L.ctx) (void)fap_ctx_destroy(L.ctx);
+  }
+  if (S->client) {
+    (void)hipStreamSynchronize(S->client);
+    (void)hipStreamDestroy(S->client);
+  }
+  if (S->d_acc) (void)hipFree(S->d_acc);
+  if (S->resident) {
+    (void)hipFree(S->resident->row_offset);
+    (void)hipFree(S->resident->edges);
+    delete S->resident;
+  }
+  if (S->arena_base) (void)hipFree(S->arena_base);
+  for (auto &kv : S->reqs) delete kv.second;
+  delete S;
+}
+
+}  // namespace
+}  // namespace fapi
+
+using namespace fapi;
+
+extern "C" {
+
+void fap_serve_default_opts(fap_serve_opts *o) {
+  if (!o) return;
+  o->hbm_budget = (int64_t)8 << 30;
+  o->stage_bytes = (int64_t)64 << 20;
+  o->lanes = 3;
+  o->max_batch = 64;
+  o->max_sources = 256;
+  o->poll_us = 20;
+}
+
+int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out) {
+  if (!out) return fail(FAP_E_ARG, "null output");
+  fap_serve_opts o;
+  fap_serve_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.hbm_budget < (1 << 20) || o.stage_bytes < (1 << 20) || o.lanes < 1 || o.lanes > 8 ||
+      o.max_batch < 1 || o.max_sources < 1 || o.poll_us < 1)
+    return fail(FAP_E_ARG, "bad server options");
+  o.stage_bytes = (int64_t)align_up((size_t)o.stage_bytes);
+  fap_server *S = new fap_server();
+  S->device = device;
+  S->opts = o;
+  S->t0 = Clock::now();
+  S->lanes.resize(o.lanes);
+  int rc = FAP_OK;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc((void **)&S->arena_base, (size_t)o.hbm_budget);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->client, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&S->d_acc, 64);
+  for (int i = 0; i < o.lanes && e == hipSuccess && rc == FAP_OK; i++) {
+    Lane &L = S->lanes[i];
+    rc = fap_ctx_create(device, &L.ctx);
+    if (rc != FAP_OK) break;
+    e = hipHostMalloc((void **)&L.h_stage, (size_t)o.stage_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&L.d_stage, (size_t)o.stage_bytes);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
+  }
+  if (e != hipSuccess || rc != FAP_OK) {
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(FAP_E_NOMEM, std::string("server allocation failed: ") + hipGetErrorString(e));
+    }
+    free_server(S);
+    return rc;
+  }
+  S->arena.init((size_t)o.hbm_budget);
+  S->st.arena_bytes = o.hbm_budget;
+  S->st.lanes = o.lanes;
+  S->thr = std::thread(serve_loop, S);
+  *out = S;
+  return FAP_OK;
+}
+
+int fap_serve_destroy(fap_server *S) {
+  if (!S) return FAP_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    S->kick++;
+  }
+  S->cv_work.notify_all();
+  if (S->thr.joinable()) S->thr.join();
+  free_server(S);
+  return FAP_OK;
+}
+
+int fap_serve_attach_graph(fap_server *S, int32_t V, int64_t E, const int32_t *ro,
+                           const int32_t *col, const int32_t *w) {
+  if (!S) return fail(FAP_E_ARG, "null server");
+  FAP_TRY(check_csr(V, E, ro, col, w));
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (auto &kv : S->reqs)
+    if (kv.second->kind == FAP_REQ_QUERY &&
+        (kv.second->state == FAP_REQ_PENDING || kv.second->state == FAP_REQ_RUNNING))
+      return fail(FAP_E_BUSY, "query requests in flight");
+  fap_graph_dev *g = nullptr;
+  FAP_TRY(fap_graph_upload(S->lanes[0].ctx, V, E, ro, col, w, &g));
+  // size every lane's SSSP state for a full wave now, so that no workspace
+  // grows (and synchronizes the device) while waves run
+  const int64_t grid = std::min<int64_t>(S->opts.max_sources, S->lanes[0].ctx->num_cus * 4);
+  for (Lane &L : S->lanes) {
+    const int rc = ensure_ws(L.ctx, L.ctx->ws_queue, (size_t)grid * 6 * (size_t)V * 4);
+    if (rc != FAP_OK) {
+      (void)fap_graph_free(S->lanes[0].ctx, g);
+      return rc;
+    }
+  }
+  if (S->resident) {
+    (void)hipFree(S->resident->row_offset);
+    (void)hipFree(S->resident->edges);
+    delete S->resident;
+  }
+  S->resident = g;
+  return FAP_OK;
+}
+
+int fap_serve_submit_graph(fap_server *S, int32_t V, int64_t E, const int32_t *ro,
+                           const int32_t *col, const int32_t *w, int64_t *id) {
+  if (!S || !id) return fail(FAP_E_ARG, "null argument");
+  FAP_TRY(check_csr(V, E, ro, col, w));
+  if ((int64_t)V * (((int64_t)V + 3) & ~3) * 4 > S->opts.hbm_budget)
+    return fail(FAP_E_NOMEM, "graph result larger than the HBM budget");
+  Request *r = new Request();
+  r->kind = FAP_REQ_GRAPH;
+  r->V = V;
+  r->E = E;
+  r->ld = ((int64_t)V + 3) & ~(int64_t)3;
+  r->row_offset.assign(ro, ro + V + 1);
+  r->edges.resize((size_t)E);
+  for (int64_t e = 0; e < E; e++) r->edges[e] = make_int2(col[e], w[e]);
+  *id = enqueue(S, r);
+  return FAP_OK;
+}
+
+int fap_serve_submit_queries(fap_server *S, int32_t count, const int32_t *src, const int32_t *dst,
+                             int64_t *id) {
+  if (!S || !id || !src || !dst) return fail(FAP_E_ARG, "null argument");
+  if (count < 1) return fail(FAP_E_ARG, "empty query batch");
+  int32_t V;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->resident) return fail(FAP_E_STATE, "no resident graph attached");
+    V = S->resident->V;
+  }
+  Request *r = new Request();
+  r->kind = FAP_REQ_QUERY;
+  r->dst.assign(dst, dst + count);
+  std::unordered_map<int32_t, int32_t> slot;
+  r->qrow.resize(count);
+  for (int32_t q = 0; q < count; q++) {
+    if (src[q] < 0 || src[q] >= V || dst[q] < 0 || dst[q] >= V) {
+      delete r;
+      return fail(FAP_E_ARG, "query vertex out of range at pair " + std::to_string(q));
+    }
+    auto it = slot.find(src[q]);
+    if (it == slot.end()) {
+      it = slot.emplace(src[q], (int32_t)r->uniq.size()).first;
+      r->uniq.push_back(src[q]);
+    }
+    r->qrow[q] = it->second;
+  }
+  *id = enqueue(S, r);
+  return FAP_OK;
+}
+
+int fap_serve_wait(fap_server *S, int64_t id, int32_t timeout_ms, fap_serve_info *info) {
+  if (!S) return fail(FAP_E_ARG, "null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(FAP_E_ARG, "unknown request id");
+  Request *r = it->second;
+  if (timeout_ms > 0)
+    S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                        [&] { return r->state >= FAP_REQ_DONE; });
+  fill_info(r, info);
+  if (r->state == FAP_REQ_FAILED) return fail(r->code != FAP_OK ? r->code : FAP_E_STATE, r->error);
+  if (r->state != FAP_REQ_DONE) return fail(FAP_E_BUSY, "request not finished");
+  return FAP_OK;
+}
+
+int fap_serve_answers(fap_server *S, int64_t id, int32_t *out) {
+  if (!S || !out) return fail(FAP_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(FAP_E_ARG, "unknown request id");
+  Request *r = it->second;
+  if (r->kind != FAP_REQ_QUERY) return fail(FAP_E_ARG, "not a query request");
+  if (r->state != FAP_REQ_DONE) return fail(FAP_E_BUSY, "request not finished");
+  memcpy(out, r->answers.data(), 4 * r->answers.size());
+  return FAP_OK;
+}
+
+int fap_serve_rows(fap_server *S, int64_t id, int32_t row0, int32_t nrows, int32_t *out) {
+  if (!S || !out) return fail(FAP_E_ARG, "null argument");
+  Request *r = nullptr;
+  FAP_TRY(pin_graph(S, id, &r));
+  int rc = FAP_OK;
+  if (row0 < 0 || nrows < 0 || (int64_t)row0 + nrows > r->V) {
+    rc = fail(FAP_E_ARG, "rows outside the result");
+  } else if (nrows > 0) {
+    std::lock_guard<std::mutex> ck(S->client_mu);
+    const int32_t *src = (const int32_t *)(S->arena_base + r->block) + (int64_t)row0 * r->ld;
+    hipError_t e = hipSetDevice(S->device);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(out, 4 * (size_t)r->V, src, 4 * (size_t)r->ld, 4 * (size_t)r->V,
+                           (size_t)nrows, hipMemcpyDeviceToHost, S->client);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->client);
+    if (e != hipSuccess) rc = hip_fail(e, "serve rows", __FILE__, __LINE__);
+  }
+  unpin(S, r);
+  return rc;
+}
+
+int fap_serve_checksum(fap_server *S, int64_t id, uint64_t *sum, uint64_t *weighted) {
+  if (!S || !sum || !weighted) return fail(FAP_E_ARG, "null argument");
+  Request *r = nullptr;
+  FAP_TRY(pin_graph(S, id, &r));
+  int rc = FAP_OK;
+  {
+    std::lock_guard<std::mutex> ck(S->client_mu);
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipSetDevice(S->device);
+    if (e == hipSuccess) rc = launch_fill_i32((int32_t *)S->d_acc, 4, 0, S->client);
+    if (e == hipSuccess && rc == FAP_OK) {
+      const int64_t cells = (int64_t)r->V * r->V;
+      const int blocks = (int)std::min<int64_t>((cells + 255) / 256, 2048);
+      hipLaunchKernelGGL(fapk::checksum2d_kernel, dim3(blocks), dim3(256), 0, S->client,
+                         (const uint32_t *)(S->arena_base + r->block), r->ld, r->V, S->d_acc);
+      e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(h, S->d_acc, sizeof(h), hipMemcpyDeviceToHost, S->client);
+      if (e == hipSuccess) e = hipStreamSynchronize(S->client);
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "serve checksum", __FILE__, __LINE__);
+    *sum = h[0];
+    *weighted = h[1];
+  }
+  unpin(S, r);
+  return rc;
+}
+
+int fap_serve_block(fap_server *S, int64_t id, int32_t **d_block, int64_t *ld, int32_t *n) {
+  if (!S || !d_block || !ld || !n) return fail(FAP_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(FAP_E_ARG, "unknown request id");
+  Request *r = it->second;
+  if (r->kind != FAP_REQ_GRAPH) return fail(FAP_E_ARG, "not a graph request");
+  if (r->state != FAP_REQ_DONE) return fail(FAP_E_BUSY, "request not finished");
+  *d_block = (int32_t *)(S->arena_base + r->block);
+  *ld = r->ld;
+  *n = r->V;
+  return FAP_OK;
+}
+
+int fap_serve_release(fap_server *S, int64_t id) {
+  if (!S) return fail(FAP_E_ARG, "null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(FAP_E_ARG, "unknown request id");
+  Request *r = it->second;
+  if (r->state == FAP_REQ_PENDING || r->state == FAP_REQ_RUNNING)
+    return fail(FAP_E_BUSY, "request still queued or running");
+  if (r->readers > 0) return fail(FAP_E_BUSY, "request being read");
+  if (r->block >= 0) S->arena.release(r->block);
+  if (r->rows >= 0) S->arena.release(r->rows);
+  S->reqs.erase(it);
+  delete r;
+  S->kick++;
+  S->cv_work.notify_all();
+  return FAP_OK;
+}
+
+int fap_serve_get_stats(fap_server *S, fap_serve_stats *out) {
+  if (!S || !out) return fail(FAP_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->st;
+  out->pending = (int64_t)S->pending.size();
+  int64_t running = 0;
+  for (const Lane &L : S->lanes) running += (int64_t)L.wave.size();
+  out->running = running;
+  out->arena_used = (int64_t)S->arena.used();
+  out->arena_peak = (int64_t)S->arena.peak();
+  if (!S->fatal.empty()) return fail(FAP_E_HIP, S->fatal);
+  return FAP_OK;
+}
+
+}  // extern "C"

@@ -46,6 +46,7 @@ enum {
   FAP_E_STATE = -4,   /* call out of order (e.g. solve before preCondition) */
   FAP_E_IO = -5,      /* file missing or malformed */
   FAP_E_DEVICE = -6,  /* a device-side bounded wait gave up (reported, not hung) */
+  FAP_E_BUSY = -7,    /* not finished yet, or still in use (serving: wait / release) */
 };
 
 typedef struct fap_ctx fap_ctx;

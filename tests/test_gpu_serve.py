@@ -1,0 +1,183 @@
+"""The request loop (include/fap_serve.h) on the GPU: whole-graph solves and
+point queries, checked bit-exact against the oracle's Dijkstra, and the same
+results alone and in a concurrent mix.  The server has no counterpart in the
+reference; the distances it returns are the reference pipeline's (K = 1:
+dense matrix + closure, fap/fap.cpp:92-181; queries: the SSSP rows of
+fap/kernel/batch_sssp.h:79-114)."""
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import fastapsp_amd as fa
+from fastapsp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rgg(n, seed):
+    u, v, w = synth.rgg_graph(n, 6, 1, 255, seed)
+    return synth.to_csr(n, u, v, w, True)
+
+
+def grid(R, seed):
+    u, v, w = synth.grid_graph(R, R, 0.1, 0.02, 1, 100, seed)
+    return synth.to_csr(R * R, u, v, w, False)
+
+
+def apsp(ro, col, w):
+    n = len(ro) - 1
+    return O.batched_sssp(np.arange(n, dtype=np.int32), ro, col, w)
+
+
+def host_checksum(D):
+    x = D.astype(np.uint32).astype(np.uint64).ravel()
+    idx = np.arange(1, x.size + 1, dtype=np.uint64)
+    return int(x.sum(dtype=np.uint64)), int((x * idx).sum(dtype=np.uint64))
+
+
+def test_serve_graphs_match_oracle():
+    sizes = [1, 2, 7, 64, 65, 130, 257, 300, 31, 500, 129, 90]
+    graphs = [rgg(n, 100 + i) for i, n in enumerate(sizes)]
+    with fa.Server(hbm_budget=256 << 20, lanes=2) as srv:
+        ids = [srv.submit_graph(*g) for g in graphs]
+        for rid, g in zip(ids, graphs):
+            info = srv.wait(rid)
+            assert info["state"] == "done" and info["kind"] == "graph"
+            D = srv.rows(rid)
+            np.testing.assert_array_equal(D, apsp(*g))
+            assert srv.checksum(rid) == host_checksum(D)
+            np.testing.assert_array_equal(srv.rows(rid, 1, 0), np.empty((0, len(g[0]) - 1)))
+            srv.release(rid)
+        st = srv.stats()
+        assert st["completed"] == len(sizes) and st["failed"] == 0
+        assert st["arena_used"] == 0 and st["arena_peak"] <= 256 << 20
+        assert st["fw_ops"] == sum(2 * n ** 3 for n in sizes)
+
+
+def test_serve_queries_match_oracle():
+    ro, col, w = grid(40, 5)
+    V = len(ro) - 1
+    rng = np.random.default_rng(7)
+    with fa.Server(hbm_budget=64 << 20, lanes=2, max_sources=48) as srv:
+        srv.attach_graph(ro, col, w)
+        batches = []
+        for k in (1, 5, 64, 17, 40):
+            src = rng.integers(0, V, k).astype(np.int32)
+            src[: k // 3] = src[0]                       # repeated sources share one row
+            dst = rng.integers(0, V, k).astype(np.int32)
+            batches.append((src, dst, srv.submit_queries(src, dst)))
+        for src, dst, rid in batches:
+            assert srv.wait(rid)["kind"] == "queries"
+            rows = O.batched_sssp(src, ro, col, w)
+            np.testing.assert_array_equal(srv.answers(rid), rows[np.arange(len(src)), dst])
+            srv.release(rid)
+        # more distinct sources than one wave may hold: refused, not hung
+        src = np.arange(60, dtype=np.int32)
+        rid = srv.submit_queries(src, src)
+        with pytest.raises(fa.FapError):
+            srv.wait(rid)
+        srv.release(rid)
+
+
+def test_serve_mix_same_as_alone():
+    """Graphs and query batches submitted from two threads at once (waves on
+    three lanes overlap); then each request again on its own.  Every result
+    is identical, and equal to the oracle."""
+    ro, col, w = grid(48, 9)
+    V = len(ro) - 1
+    rng = np.random.default_rng(3)
+    sizes = synth.loguniform_sizes(10, 40, 600, 4)
+    graphs = [rgg(int(n), 300 + i) for i, n in enumerate(sizes)]
+    queries = []
+    for i in range(10):
+        k = int(rng.integers(1, 65))
+        queries.append((rng.integers(0, V, k).astype(np.int32),
+                        rng.integers(0, V, k).astype(np.int32)))
+    with fa.Server(hbm_budget=512 << 20, lanes=3) as srv:
+        srv.attach_graph(ro, col, w)
+        g_ids, q_ids = [None] * len(graphs), [None] * len(queries)
+
+        def push_graphs():
+            for i, g in enumerate(graphs):
+                g_ids[i] = srv.submit_graph(*g)
+                time.sleep(0.001 * (i % 3))
+
+        def push_queries():
+            for i, (s, d) in enumerate(queries):
+                q_ids[i] = srv.submit_queries(s, d)
+                time.sleep(0.001 * (i % 2))
+
+        ts = [threading.Thread(target=push_graphs), threading.Thread(target=push_queries)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        mix_g, mix_q = [], []
+        for rid in g_ids:
+            srv.wait(rid)
+            mix_g.append(srv.rows(rid))
+            srv.release(rid)
+        for rid in q_ids:
+            srv.wait(rid)
+            mix_q.append(srv.answers(rid))
+            srv.release(rid)
+        assert srv.stats()["waves"] >= 2
+        for g, D in zip(graphs, mix_g):
+            rid = srv.submit_graph(*g)
+            info = srv.wait(rid)
+            assert info["wave_size"] == 1
+            np.testing.assert_array_equal(srv.rows(rid), D)
+            srv.release(rid)
+            np.testing.assert_array_equal(D, apsp(*g))
+        for (s, d), a in zip(queries, mix_q):
+            rid = srv.submit_queries(s, d)
+            srv.wait(rid)
+            np.testing.assert_array_equal(srv.answers(rid), a)
+            srv.release(rid)
+            np.testing.assert_array_equal(a, O.batched_sssp(s, ro, col, w)[np.arange(len(s)), d])
+
+
+def test_serve_admission_by_budget():
+    """A 2 MiB arena holds three 400-vertex results at most (640 KB each):
+    later graphs wait until earlier ones are released, none fails."""
+    graphs = [rgg(400, 500 + i) for i in range(8)]
+    with fa.Server(hbm_budget=2 << 20, lanes=2) as srv:
+        ids = [srv.submit_graph(*g) for g in graphs]
+        for rid, g in zip(ids, graphs):
+            srv.wait(rid)
+            np.testing.assert_array_equal(srv.rows(rid, 10, 5), apsp(*g)[10:15])
+            srv.release(rid)
+        st = srv.stats()
+        assert st["completed"] == 8 and st["failed"] == 0
+        assert st["arena_peak"] <= 2 << 20
+        # larger than the whole budget: refused at submit
+        with pytest.raises(fa.FapError):
+            srv.submit_graph(*rgg(800, 1))
+
+
+def test_serve_rejects_bad_requests():
+    with fa.Server(hbm_budget=16 << 20, lanes=1) as srv:
+        with pytest.raises(fa.FapError):
+            srv.submit_queries([0], [1])                 # no resident graph yet
+        ro, col, w = grid(10, 1)
+        srv.attach_graph(ro, col, w)
+        with pytest.raises(fa.FapError):
+            srv.submit_queries([0], [100])               # vertex out of range
+        with pytest.raises(fa.FapError):
+            srv.submit_graph([0, 1], [5], [1])           # arc to a vertex that does not exist
+        with pytest.raises(fa.FapError):
+            srv.submit_graph([0, 1], [0], [-1])          # negative weight
+        with pytest.raises(fa.FapError):
+            srv.wait(12345)                              # unknown id
+        rid = srv.submit_queries([3], [3])
+        srv.wait(rid)
+        assert srv.answers(rid)[0] == 0
+        with pytest.raises(fa.FapError):
+            srv.rows(rid)                                # not a graph request
+        srv.release(rid)
+        with pytest.raises(fa.FapError):
+            srv.release(rid)                             # already released

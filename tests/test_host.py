@@ -21,7 +21,7 @@ REF_GRAPH_DIR = "/root/reference/app"
 
 def declared_symbols():
     names = set()
-    for h in ("fap_dev.h", "fap_graph.h", "fap_ops.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fap\w*)\s*\(", txt, flags=re.M):
