@@ -86,6 +86,7 @@ def bench_dense(args):
     B = synth.dense_matrix(n, 0.01, 1, 1000, 22)
     log(f"[bench] generated in {time.time() - t0:.1f}s")
     ctx = fa.Context(0)
+    tuned = _tune(ctx, args, "fw", fa.TuneCache.key_fw(n), lambda: ctx.tune_fw(n, reps=1))
     d_src = ctx.to_device(M0)
     d_fw = ctx.alloc((n, n))
     dA, dB = ctx.to_device(A), ctx.to_device(B)
@@ -156,9 +157,29 @@ def bench_dense(args):
                         "unit": "TOP/s", "frac": round(ach_fw / VALU_PEAK_TOPS, 3),
                         "ms_per_closure": round(fw_avg, 3)},
         "cpu_baseline": cpu,
+        "tuned": tuned,
     }
     print(json.dumps(res), flush=True)
     return 0
+
+
+def _tune(ctx, args, kind, key, run):
+    """Attach a tune cache (--tune-cache file if given), measure the bucket if
+    it has no entry, save the file back; returns the chosen parameters."""
+    import os
+    import fastapsp_amd as fa
+    path = args.tune_cache
+    cache = fa.TuneCache(path if path and os.path.exists(path) else None)
+    ctx.set_tune_cache(cache)
+    hit = cache.get(kind, key)
+    if hit is None:
+        t0 = time.perf_counter()
+        run()
+        hit = cache.get(kind, key)
+        log(f"[bench] tuned {kind} bucket {key} in {time.perf_counter() - t0:.2f}s: {hit}")
+        if path:
+            cache.save(path)
+    return {"kind": kind, "bucket": int(key), "params": list(hit[0]), "ms": round(hit[1], 3)}
 
 
 def bench_copy(args):
@@ -629,6 +650,8 @@ def bench_batch(args):
         for p_, n, ld in zip(ptrs, sizes, lds):
             ctx.fw(p_, int(n), int(ld), ctx.stream)
 
+    tuned = _tune(ctx, args, "batch", fa.TuneCache.key_batch(sizes),
+                  lambda: ctx.tune_batch(sizes, reps=1))
     plan = ctx.fw_plan(ptrs, [int(x) for x in sizes], lds, use_graph=True)
     info = plan.info()
 
@@ -667,6 +690,7 @@ def bench_batch(args):
         "batched_launches": info["launches"], "batched_tasks": info["tasks"],
         "speedup_batched_vs_plain": round(res["plain"] / res["batched"], 2),
         "plain_and_batched_identical": bool(same),
+        "tuned": tuned,
     }
     print(json.dumps(out), flush=True)
     plan.close()
@@ -695,6 +719,8 @@ def main():
     ap.add_argument("--rate-frac", type=float, default=0.5)
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--hbm-gib", type=float, default=16.0)
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: loaded if present, missing buckets tuned and saved")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the parts of a "

@@ -150,6 +150,22 @@ _SIGS = {
     "fap_serve_block": (_int, [_vp, _i64, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
     "fap_serve_release": (_int, [_vp, _i64]),
     "fap_serve_get_stats": (_int, [_vp, C.POINTER(ServeStats)]),
+    # autotuner (include/fap_tune.h)
+    "fap_tune_cache_create": (_int, [C.POINTER(_vp)]),
+    "fap_tune_cache_destroy": (_int, [_vp]),
+    "fap_tune_cache_load": (_int, [_vp, C.c_char_p]),
+    "fap_tune_cache_save": (_int, [_vp, C.c_char_p]),
+    "fap_tune_cache_size": (_int, [_vp, _pi32]),
+    "fap_tune_cache_set": (_int, [_vp, C.c_char_p, _i64, _pi32, _i32, C.c_double]),
+    "fap_tune_cache_get": (_int, [_vp, C.c_char_p, _i64, _pi32, _i32,
+                                  C.POINTER(C.c_double)]),
+    "fap_tune_key_fw": (_i64, [_i32]),
+    "fap_tune_key_sssp": (_i64, [_i32, _i64]),
+    "fap_tune_key_batch": (_i64, [_i32, _pi32]),
+    "fap_ctx_set_tune_cache": (_int, [_vp, _vp]),
+    "fap_tune_fw": (_int, [_vp, _i32, _i32]),
+    "fap_tune_sssp": (_int, [_vp, _vp, _pi32, _i32, _i32]),
+    "fap_tune_batch": (_int, [_vp, _i32, _pi32, _i32]),
 }
 
 _lib = None
@@ -232,6 +248,70 @@ class DeviceArray:
             self.ptr = 0
 
 
+class TuneCache:
+    """The autotuner's cache (fap_tune_cache): host-side, no device needed."""
+
+    _NP = {"fw": 2, "sssp": 2, "batch": 1}
+
+    def __init__(self, path: str | None = None):
+        h = _vp()
+        check(load().fap_tune_cache_create(C.byref(h)), "fap_tune_cache_create")
+        self.handle = h.value
+        if path:
+            self.load(path)
+
+    def load(self, path: str):
+        check(load().fap_tune_cache_load(self.handle, path.encode()), "tune cache load")
+
+    def save(self, path: str):
+        check(load().fap_tune_cache_save(self.handle, path.encode()), "tune cache save")
+
+    def __len__(self):
+        n = C.c_int32()
+        check(load().fap_tune_cache_size(self.handle, C.byref(n)))
+        return n.value
+
+    def set(self, kind: str, key: int, params, ms: float):
+        p = np.ascontiguousarray(params, dtype=np.int32)
+        check(load().fap_tune_cache_set(self.handle, kind.encode(), int(key), _p32(p), p.size,
+                                        float(ms)), "tune cache set")
+
+    def get(self, kind: str, key: int):
+        """(params, ms), or None when the bucket has no entry."""
+        p = np.zeros(self._NP.get(kind, 0), np.int32)
+        ms = C.c_double()
+        rc = load().fap_tune_cache_get(self.handle, kind.encode(), int(key), _p32(p), p.size,
+                                       C.byref(ms))
+        if rc == -4:
+            return None
+        check(rc, "tune cache get")
+        return tuple(int(x) for x in p), ms.value
+
+    @staticmethod
+    def key_fw(n: int) -> int:
+        return load().fap_tune_key_fw(int(n))
+
+    @staticmethod
+    def key_sssp(V: int, E: int) -> int:
+        return load().fap_tune_key_sssp(int(V), int(E))
+
+    @staticmethod
+    def key_batch(sizes) -> int:
+        n = np.ascontiguousarray(sizes, dtype=np.int32)
+        return load().fap_tune_key_batch(n.size, _p32(n))
+
+    def close(self):
+        if self.handle:
+            load().fap_tune_cache_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """A device context (fap_ctx): stream, workspaces, device memory."""
 
@@ -265,6 +345,25 @@ class Context:
         v = C.c_int64()
         check(load().fap_ctx_get_option(self.handle, name.encode(), C.byref(v)), "get_option")
         return v.value
+
+    def set_tune_cache(self, cache: "TuneCache | None"):
+        """Consult `cache` for every option left at auto (None detaches)."""
+        check(load().fap_ctx_set_tune_cache(self.handle,
+                                          cache.handle if cache is not None else None),
+              "set_tune_cache")
+        self._tune = cache  # keep it alive while attached
+
+    def tune_fw(self, n: int, reps: int = 2):
+        check(load().fap_tune_fw(self.handle, int(n), int(reps)), "tune_fw")
+
+    def tune_sssp(self, graph: "DeviceGraph", sources, reps: int = 1):
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        check(load().fap_tune_sssp(self.handle, graph.handle, _p32(s), s.size, int(reps)),
+              "tune_sssp")
+
+    def tune_batch(self, sizes, reps: int = 2):
+        n = np.ascontiguousarray(sizes, dtype=np.int32)
+        check(load().fap_tune_batch(self.handle, n.size, _p32(n), int(reps)), "tune_batch")
 
     def alloc(self, shape) -> DeviceArray:
         shape = (shape,) if isinstance(shape, (int, np.integer)) else tuple(shape)

@@ -159,11 +159,12 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
     out->launch_tasks.push_back(first);
     out->seg_off.push_back((int32_t)out->segs.size());
   }
+  out->key = fap_tune_key_batch(count, n);
 }
 
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
                     const fapk::BatchSeg *d_segs, hipStream_t s) {
-  const int cap = ctx->num_cus * 4;
+  const int cap = ctx->num_cus * batch_grid_per_cu(ctx, desc.key);
   for (size_t p = 0; p < desc.launch_tasks.size(); p++) {
     const int cnt = desc.launch_tasks[p];
     const int nseg = desc.seg_off[p + 1] - desc.seg_off[p];

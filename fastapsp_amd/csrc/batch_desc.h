@@ -59,6 +59,7 @@ struct FwBatchDesc {
   std::vector<int32_t> seg_off;
   std::vector<int32_t> launch_tasks;
   int64_t ops = 0;
+  int64_t key = 0;  // tune cache bucket (fap_tune_key_batch)
 };
 
 void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fap_dev.h"
+#include "fap_tune.h"
 
 namespace fapi {
 
@@ -84,11 +85,14 @@ struct fap_ctx {
   hipStream_t aux_stream = nullptr;
   hipEvent_t ev_fw_la = nullptr;
   hipEvent_t ev_fw_panel = nullptr;
-  // tuning options (fap_ctx_set_option); they change speed, never results
-  int opt_fw_lookahead = 0;      // run round r+1's phases 1-2 beside round r's phase 3
-  int opt_fw_tile = 0;           // FW block size 64 / 128; 0 = by n
-  int opt_sssp_threads = 1024;   // threads per SSSP workgroup: 256, 512, 1024
-  int opt_sssp_delta = 0;        // near-far band; 0 = per graph (8 x mean weight)
+  // tuning options (fap_ctx_set_option); they change speed, never results.
+  // "auto" values defer to the tune cache, then to the defaults.
+  int opt_fw_lookahead = -1;     // run round r+1's phases 1-2 beside round r's phase 3; -1 auto
+  int opt_fw_tile = 0;           // FW block size 64 / 128; 0 auto
+  int opt_sssp_threads = 0;      // threads per SSSP workgroup: 256, 512, 1024; 0 auto
+  int opt_sssp_delta = 0;        // near-far band; 0 auto (graph's own, cache, 8 x mean weight)
+  int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
+  fap_tune_cache *tune = nullptr;  // not owned
 };
 
 namespace fapi {
@@ -98,6 +102,13 @@ int ensure_ws(fap_ctx *ctx, Workspace &ws, size_t bytes);
 inline hipStream_t pick_stream(fap_ctx *ctx, void *stream) {
   return stream ? (hipStream_t)stream : ctx->stream;
 }
+// Effective tuning settings: the context option when set, else the tune
+// cache entry for the bucket, else the default (tune_cache.cpp).
+bool tune_lookup(const fap_ctx *ctx, const char *kind, int64_t key, int32_t *params);
+fap_tune_cache *tune_env_cache();
+void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead);
+void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int *threads, int *delta);
+int batch_grid_per_cu(const fap_ctx *ctx, int64_t batch_key);
 // Device-side helpers implemented in ctx.hip.
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);

@@ -160,6 +160,7 @@ int fap_ctx_create(int device, fap_ctx **out) {
     delete c;
     return hip_fail(e, "context setup", __FILE__, __LINE__);
   }
+  c->tune = tune_env_cache();
   *out = c;
   return FAP_OK;
 }
@@ -186,18 +187,22 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   if (!ctx || !name) return fail(FAP_E_ARG, "null argument");
   const std::string k(name);
   if (k == "fw_lookahead") {
-    ctx->opt_fw_lookahead = value != 0;
+    if (value < -1 || value > 1) return fail(FAP_E_ARG, "fw_lookahead must be -1 (auto), 0 or 1");
+    ctx->opt_fw_lookahead = (int)value;
   } else if (k == "fw_tile") {
     if (value != 0 && value != 64 && value != 128)
       return fail(FAP_E_ARG, "fw_tile must be 0, 64 or 128");
     ctx->opt_fw_tile = (int)value;
   } else if (k == "sssp_threads") {
-    if (value != 256 && value != 512 && value != 1024)
-      return fail(FAP_E_ARG, "sssp_threads must be 256, 512 or 1024");
+    if (value != 0 && value != 256 && value != 512 && value != 1024)
+      return fail(FAP_E_ARG, "sssp_threads must be 0 (auto), 256, 512 or 1024");
     ctx->opt_sssp_threads = (int)value;
   } else if (k == "sssp_delta") {
     if (value < 0 || value > INT32_MAX / 4) return fail(FAP_E_ARG, "sssp_delta out of range");
     ctx->opt_sssp_delta = (int)value;
+  } else if (k == "batch_grid") {
+    if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
+    ctx->opt_batch_grid = (int)value;
   } else {
     return fail(FAP_E_ARG, "unknown option '" + k + "'");
   }
@@ -211,6 +216,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "fw_tile") *value = ctx->opt_fw_tile;
   else if (k == "sssp_threads") *value = ctx->opt_sssp_threads;
   else if (k == "sssp_delta") *value = ctx->opt_sssp_delta;
+  else if (k == "batch_grid") *value = ctx->opt_batch_grid;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

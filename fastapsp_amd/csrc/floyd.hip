@@ -219,11 +219,12 @@ struct Fw {
   // r+1 first (mode 1), then, on the auxiliary stream, round r+1's diagonal
   // and panels run while the main stream finishes round r's other tiles
   // (mode 2).  All work is ordered before later work on `s` on return.
-  static int run(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
+  static int run(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, bool lookahead,
+                 hipStream_t s) {
     const int nb = (n + T - 1) / T;
     const int flags = vec_flags(d, ld, d, ld, d, ld);
     FAP_TRY(diag_panel(d, n, ld, 0, nb, flags, s));
-    const bool overlap = ctx && ctx->opt_fw_lookahead && ctx->aux_stream && nb >= 3;
+    const bool overlap = lookahead && ctx && ctx->aux_stream && nb >= 3;
     for (int r = 0; r < nb; r++) {
       if (r + 1 < nb && overlap) {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 1, s));
@@ -244,20 +245,17 @@ struct Fw {
 
 }  // namespace
 
-// Block size: the context option if set, else 64.  Measured on MI355X
-// (bench.py --workload fwsweep), 64 beats 128 from n = 256 to n = 8192: 4x
-// more tiles per launch and a diagonal phase that is 8x cheaper outweigh
-// twice the rounds.
-int fw_block_size(const fap_ctx *ctx, int32_t n) {
-  (void)n;
-  if (ctx && (ctx->opt_fw_tile == 64 || ctx->opt_fw_tile == 128)) return ctx->opt_fw_tile;
-  return 64;
-}
-
+// Block size and look-ahead: the context options if set, else the tune
+// cache, else 64 without look-ahead.  Measured on MI355X, 64 beats 128 from
+// n = 256 to n = 8192: 4x more tiles per launch and a diagonal phase that
+// is 8x cheaper outweigh twice the rounds.
 int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
   if (n <= 0) return FAP_OK;
-  if (fw_block_size(ctx, n) == 128) return Fw<8>::run(ctx, d, n, ld, s);
-  return Fw<4>::run(ctx, d, n, ld, s);
+  int tile;
+  bool lookahead;
+  fw_settings(ctx, n, &tile, &lookahead);
+  if (tile == 128) return Fw<8>::run(ctx, d, n, ld, lookahead, s);
+  return Fw<4>::run(ctx, d, n, ld, lookahead, s);
 }
 
 }  // namespace fapi

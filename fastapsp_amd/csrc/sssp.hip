@@ -209,11 +209,6 @@ namespace fapi {
 // Threshold width: 8 mean arc weights, the best of 1..32 x mean measured on
 // the 512 x 512 road-like grid (w 1..100) and the 64 x 64 unit grid
 // (bench.py --workload ssspsweep).
-static int pick_delta(const fap_graph_dev *g) {
-  if (g->E == 0) return 1;
-  const double mean = (double)g->weight_sum / (double)g->E;
-  return (int)std::max(1.0, std::min(1.0e6, 8.0 * mean));
-}
 
 // Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
 // pointers) when d_rowptr is not null.
@@ -226,15 +221,14 @@ int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources
   // 2 per CU and a 4 GiB state budget
   const int64_t stride = 6 * (int64_t)V;
   const int64_t by_mem = std::max<int64_t>(1, ((int64_t)4 << 30) / (stride * 4));
-  const int per_cu = ctx->opt_sssp_threads == 256 ? 4 : ctx->opt_sssp_threads == 512 ? 2 : 1;
+  int threads, delta;
+  sssp_settings(ctx, g, &threads, &delta);
+  const int per_cu = threads == 256 ? 4 : threads == 512 ? 2 : 1;
   const int grid = (int)std::max<int64_t>(
       1, std::min<int64_t>({(int64_t)n_src, (int64_t)ctx->num_cus * per_cu, by_mem}));
   FAP_TRY(ensure_ws(ctx, ctx->ws_queue, (size_t)grid * stride * sizeof(uint32_t)));
-  const int delta = ctx->opt_sssp_delta > 0 ? ctx->opt_sssp_delta
-                    : g->delta > 0          ? g->delta
-                                            : pick_delta(g);
   uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
-  switch (ctx->opt_sssp_threads) {
+  switch (threads) {
     case 256:
       hipLaunchKernelGGL(fapk::sssp_nf_kernel<256>, dim3(grid), dim3(256), 0, s, g->row_offset,
                          g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,

@@ -62,11 +62,14 @@ int fap_device_count(int *count);
 int fap_ctx_create(int device, fap_ctx **out);
 int fap_ctx_destroy(fap_ctx *ctx);
 int fap_ctx_stream(fap_ctx *ctx, void **stream);
-/* Tuning options; they change speed, never results:
- *   "fw_lookahead" 0/1      overlap FW round r+1's diagonal/panels with round r
- *   "fw_tile"      0/64/128 FW block size (0 = 64)
- *   "sssp_threads" 256/512/1024  threads per SSSP workgroup (one source each)
- *   "sssp_delta"   >= 0     near-far band width (0 = per graph, 8 x mean weight) */
+/* Tuning options; they change speed, never results.  "Auto" (0, or -1 for
+ * fw_lookahead; the defaults) defers to the context's tune cache
+ * (fap_tune.h), then to the default in brackets:
+ *   "fw_lookahead" -1/0/1   overlap FW round r+1's diagonal/panels with round r [0]
+ *   "fw_tile"      0/64/128 FW block size [64]
+ *   "sssp_threads" 0/256/512/1024  threads per SSSP workgroup (one source each) [1024]
+ *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 x mean weight]
+ *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);

@@ -118,7 +118,7 @@ def fw_config(gpu_ctx, request):
     gpu_ctx.set_option("fw_lookahead", look)
     yield request.param
     gpu_ctx.set_option("fw_tile", 0)
-    gpu_ctx.set_option("fw_lookahead", 0)
+    gpu_ctx.set_option("fw_lookahead", -1)
 
 
 FW_CONFIGS = [(64, 0), (128, 0), (64, 1), (128, 1)]
@@ -177,7 +177,7 @@ def sssp_config(gpu_ctx, request):
     gpu_ctx.set_option("sssp_threads", threads)
     gpu_ctx.set_option("sssp_delta", delta)
     yield request.param
-    gpu_ctx.set_option("sssp_threads", 1024)
+    gpu_ctx.set_option("sssp_threads", 0)
     gpu_ctx.set_option("sssp_delta", 0)
 
 
