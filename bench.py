@@ -579,41 +579,48 @@ def bench_fw_sweep(args):
 
 
 def bench_sssp_sweep(args):
-    """Multi-source SSSP time per (threads, delta) on one part's boundary of
-    the 512x512 grid (w 1..100) and of the 64x64 unit grid."""
+    """Multi-source SSSP time per (threads, sources per workgroup pass) on the
+    512x512 grid (w 1..100, K=256 partition): the boundary vertices of one
+    part (plain per-part stage 1) and of 32 parts (a batched group)."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
     lib = fa.load()
     ctx = fa.Context(0)
-    for (R, drop, diag, wl, wh, seed, K) in [(512, 0.10, 0.02, 1, 100, 3, 256),
-                                             (64, 0.05, 0.0, 1, 1, 1, 8)]:
-        V = R * R
-        u, v, w = synth.grid_graph(R, R, drop, diag, wl, wh, seed)
-        ro, col, wt = synth.to_csr(V, u, v, w, False)
-        g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
-        g.preCondition()
-        vn, vo, bn, _ = g.part_info()
-        st2ed = g.getMapping()
-        g.close()
-        srcs = st2ed[int(vo[1]): int(vo[1]) + int(bn[1])]
-        dg = ctx.upload_graph(ro, col, wt)
+    R, K = args.grid, args.parts_k
+    V = R * R
+    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
+    g.preCondition()
+    vn, vo, bn, _ = g.part_info()
+    st2ed = g.getMapping()
+    g.close()
+    dg = ctx.upload_graph(ro, col, wt)
+    for parts in (1, 32):
+        srcs = np.concatenate([st2ed[int(vo[p]): int(vo[p]) + int(bn[p])]
+                               for p in range(1, parts + 1)]).astype(np.int32)
         dsrc = ctx.to_device(srcs)
         out = ctx.alloc((len(srcs), V))
-        mean_w = float(wt.mean())
+        ref = None
         for threads in (256, 512, 1024):
-            for mult in (1, 2, 4, 8, 32):
-                delta = max(1, int(round(mult * mean_w)))
+            for group in (1, 4, 8):
                 ctx.set_option("sssp_threads", threads)
-                ctx.set_option("sssp_delta", delta)
+                ctx.set_option("sssp_group", group)
                 ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V,
                                                         ctx.stream), reps=1)
-                r = {"graph": f"{R}x{R} w{wl}..{wh}", "sources": len(srcs), "threads": threads,
-                     "delta": delta, "ms": round(ms, 3),
-                     "edges_per_s": len(srcs) * len(col) / (ms * 1e-3)}
+                cs = ctx.checksum(out.ptr, len(srcs) * V)
+                ref = cs if ref is None else ref
+                byts = 4.0 * (V + 1) + 8.0 * len(col) + 4.0 * len(srcs) * V
+                r = {"graph": f"{R}x{R} w1..100", "parts": parts, "sources": len(srcs),
+                     "threads": threads, "group": group, "ms": round(ms, 3),
+                     "edges_per_s": len(srcs) * len(col) / (ms * 1e-3),
+                     "algorithmic_GB_per_s": round(byts / (ms * 1e-3) / 1e9, 1),
+                     "same_rows": cs == ref}
                 print(json.dumps(r), flush=True)
-        dg.close()
-    ctx.set_option("sssp_threads", 1024)
-    ctx.set_option("sssp_delta", 0)
+        out.free()
+    dg.close()
+    ctx.set_option("sssp_threads", 0)
+    ctx.set_option("sssp_group", 0)
     return 0
 
 

@@ -134,6 +134,12 @@ _SIGS = {
     "fapg_get_distance_from_one_point": (_int, [_vp, _i32, _pf32]),
     "fapg_distance_p2p": (_int, [_vp, _i32, _i32, _pi32]),
     "fapg_subgraph_checksum": (_int, [_vp, _i32, _pu64, _pu64]),
+    "fapg_get_subgraph_path": (_int, [_vp, _i32, _pi32]),
+    "fapg_get_all_path": (_int, [_vp, _pi32]),
+    "fapg_get_path_from_one_point": (_int, [_vp, _i32, _pi32]),
+    "fapg_path_p2p": (_int, [_vp, _i32, _i32, _pi32, _i32, _pi32]),
+    "fap_pred_rows": (_int, [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _i64, _vp]),
+    "fap_subgraph_pred": (_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
     # serving (include/fap_serve.h)
     "fap_serve_default_opts": (None, [C.POINTER(ServeOpts)]),
@@ -497,6 +503,14 @@ class DeviceGraph:
         check(load().fap_subgraph_solve_batch(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
                                               _p32(B), R, ld_rb, stream),
               "fap_subgraph_solve_batch")
+
+    def pred_rows(self, d_dist, ld, d_row_vertex, nrows, d_pred, ld_pred, stream=None):
+        check(load().fap_pred_rows(self.ctx.handle, self.handle, d_dist, ld, d_row_vertex, nrows,
+                                   d_pred, ld_pred, stream), "fap_pred_rows")
+
+    def subgraph_pred(self, start, n_sub, d_rows, ld, d_pred, ld_pred, stream=None):
+        check(load().fap_subgraph_pred(self.ctx.handle, self.handle, start, n_sub, d_rows, ld,
+                                       d_pred, ld_pred, stream), "fap_subgraph_pred")
 
     def close(self):
         if self.handle:

@@ -250,6 +250,38 @@ int fapg_distance_p2p(fapg *h, int32_t src, int32_t dst, int32_t *out) {
   return FAP_OK;
 }
 
+int fapg_get_subgraph_path(fapg *h, int32_t id, int32_t *out) {
+  FAPG_CHECK(h);
+  std::vector<int32_t> v = h->g->getSubGraphPath(id);
+  if (v.empty() && h->g->status() != 0) return from_status(h->g);
+  return copy_out(v, out, "path");
+}
+
+int fapg_get_all_path(fapg *h, int32_t *out) {
+  FAPG_CHECK(h);
+  std::vector<int32_t> v = h->g->getAllPath();
+  if (v.empty() && h->g->status() != 0) return from_status(h->g);
+  return copy_out(v, out, "path");
+}
+
+int fapg_get_path_from_one_point(fapg *h, int32_t v, int32_t *out) {
+  FAPG_CHECK(h);
+  std::vector<int32_t> r = h->g->getPathFromOnePoint(v);
+  if (r.empty()) return from_status(h->g);
+  return copy_out(r, out, "row");
+}
+
+int fapg_path_p2p(fapg *h, int32_t src, int32_t dst, int32_t *out, int32_t cap, int32_t *len) {
+  FAPG_CHECK(h);
+  if (!len || cap < 0 || (cap > 0 && !out)) return fail(FAP_E_ARG, "bad output");
+  std::vector<int32_t> seq = h->g->getPathP2P(src, dst);
+  if (h->g->status() != 0) return from_status(h->g);
+  *len = (int32_t)seq.size();
+  if ((int32_t)seq.size() > cap) return fail(FAP_E_ARG, "output too small for the path");
+  if (!seq.empty()) memcpy(out, seq.data(), seq.size() * sizeof(int32_t));
+  return FAP_OK;
+}
+
 int fapg_subgraph_checksum(fapg *h, int32_t id, uint64_t *sum, uint64_t *weighted) {
   FAPG_CHECK(h);
   if (!sum || !weighted) return fail(FAP_E_ARG, "null output");

@@ -60,6 +60,8 @@ struct fap_graph_dev {
   int32_t max_degree = 0;
   int64_t weight_sum = 0;
   int32_t delta = 0;  // SSSP near-far band width (0 = derived from the mean weight)
+  int32_t *in_offset = nullptr;  // transposed CSR (V+1), built on first path request
+  int2 *in_arcs = nullptr;       // E x {source, w}
   std::vector<int32_t> h_st2ed;     // host copy of st2ed (source lists of batched solves)
   std::vector<int32_t> h_graph_id;  // host copy of graph_id (part range checks)
 };
@@ -91,6 +93,7 @@ struct fap_ctx {
   int opt_fw_tile = 0;           // FW block size 64 / 128; 0 auto
   int opt_sssp_threads = 0;      // threads per SSSP workgroup: 256, 512, 1024; 0 auto
   int opt_sssp_delta = 0;        // near-far band; 0 auto (graph's own, cache, 8 x mean weight)
+  int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 auto
   int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
   fap_tune_cache *tune = nullptr;  // not owned
 };
@@ -107,7 +110,8 @@ inline hipStream_t pick_stream(fap_ctx *ctx, void *stream) {
 bool tune_lookup(const fap_ctx *ctx, const char *kind, int64_t key, int32_t *params);
 fap_tune_cache *tune_env_cache();
 void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead);
-void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int *threads, int *delta);
+void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
+                   int *delta, int *group);
 int batch_grid_per_cu(const fap_ctx *ctx, int64_t batch_key);
 // Device-side helpers implemented in ctx.hip.
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);

@@ -200,6 +200,10 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "sssp_delta") {
     if (value < 0 || value > INT32_MAX / 4) return fail(FAP_E_ARG, "sssp_delta out of range");
     ctx->opt_sssp_delta = (int)value;
+  } else if (k == "sssp_group") {
+    if (value != 0 && value != 1 && value != 4 && value != 8)
+      return fail(FAP_E_ARG, "sssp_group must be 0 (auto), 1, 4 or 8");
+    ctx->opt_sssp_group = (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
     ctx->opt_batch_grid = (int)value;
@@ -217,6 +221,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_threads") *value = ctx->opt_sssp_threads;
   else if (k == "sssp_delta") *value = ctx->opt_sssp_delta;
   else if (k == "batch_grid") *value = ctx->opt_batch_grid;
+  else if (k == "sssp_group") *value = ctx->opt_sssp_group;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }
@@ -442,10 +447,11 @@ int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g, const int32_t *graph
 int fap_graph_free(fap_ctx *ctx, fap_graph_dev *g) {
   if (!g) return FAP_OK;
   if (ctx) (void)hipSetDevice(ctx->device);
-  int32_t *bufs[] = {g->row_offset, g->graph_id, g->st2ed, g->ed2st};
+  int32_t *bufs[] = {g->row_offset, g->graph_id, g->st2ed, g->ed2st, g->in_offset};
   for (int32_t *b : bufs)
     if (b) (void)hipFree(b);
   if (g->edges) (void)hipFree(g->edges);
+  if (g->in_arcs) (void)hipFree(g->in_arcs);
   delete g;
   return FAP_OK;
 }

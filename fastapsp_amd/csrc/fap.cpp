@@ -43,12 +43,30 @@ struct fapGraph::Device {
   int32_t *d_multi = nullptr;
   size_t multi_bytes = 0;
   std::unordered_map<int32_t, int64_t> multi_off;  // part id -> row offset in d_multi
+  // predecessor rows beside each distance buffer (is_path_needed)
+  int32_t *d_block_path = nullptr, *d_all_path = nullptr, *d_multi_path = nullptr;
+  size_t block_path_bytes = 0, all_path_bytes = 0, multi_path_bytes = 0;
+  bool block_path_valid = false, all_path_valid = false, multi_path_valid = false;
+
+  // grows a device buffer to `need` bytes (contents not kept)
+  int grow(int32_t *&p, size_t &have, size_t need) {
+    if (have >= need) return FAP_OK;
+    if (p) fap_dev_free(ctx, p);
+    p = nullptr;
+    have = 0;
+    void *q = nullptr;
+    const int rc = fap_dev_alloc(ctx, need, &q);
+    if (rc != FAP_OK) return rc;
+    p = (int32_t *)q;
+    have = need;
+    return FAP_OK;
+  }
 
   ~Device() {
     if (ctx) {
-      if (d_block) fap_dev_free(ctx, d_block);
-      if (d_all) fap_dev_free(ctx, d_all);
-      if (d_multi) fap_dev_free(ctx, d_multi);
+      int32_t *bufs[] = {d_block, d_all, d_multi, d_block_path, d_all_path, d_multi_path};
+      for (int32_t *b : bufs)
+        if (b) fap_dev_free(ctx, b);
       if (g) fap_graph_free(ctx, g);
       fap_ctx_destroy(ctx);
     }
@@ -320,6 +338,16 @@ int32_t fapGraph::run_batch(const std::vector<int32_t> &ids, const std::vector<i
   return 0;
 }
 
+// Predecessor rows of one solved part block (fap_subgraph_pred).
+int32_t fapGraph::run_pred(int32_t id, const int32_t *d_rowblock, int32_t *d_predblock) {
+  if (C_BlockVer_num[id] == 0) return 0;
+  if (fap_subgraph_pred(dev_->ctx, dev_->g, C_BlockVer_offset[id], C_BlockVer_num[id], d_rowblock,
+                        num_vertexs, d_predblock, num_vertexs, nullptr) != FAP_OK)
+    return fail(FAP_E_HIP, std::string("paths of part ") + std::to_string(id) + ": " +
+                               fap_last_error());
+  return 0;
+}
+
 // fap/fap.cpp:184-194.  The full V x V matrix, parts stacked in part order.
 int32_t fapGraph::solve(bool is_path_needed_) {
   reset_status();
@@ -352,12 +380,22 @@ int32_t fapGraph::solve(bool is_path_needed_) {
     }
   }
   if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
+  d.all_path_valid = false;
+  if (is_path_needed) {
+    if (d.grow(d.d_all_path, d.all_path_bytes, bytes) != FAP_OK)
+      return fail(FAP_E_NOMEM, std::string("the V x V path matrix does not fit: ") +
+                                   fap_last_error());
+    for (int32_t i = 1; i <= K; i++) {
+      const int64_t off = (int64_t)C_BlockVer_offset[i] * num_vertexs;
+      if (run_pred(i, d.d_all + off, d.d_all_path + off) != 0) return -1;
+    }
+    d.all_path_valid = true;
+  }
   full_solved_ = true;
   d.all_host_valid = false;
   current_subgraph_id = K;
   status_ = 0;
   error_.clear();
-  if (is_path_needed) error_ = "path matrices are not produced (is_path_needed ignored)";
   return 0;
 }
 
@@ -380,14 +418,20 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool is_path_needed_) {
     d.block_bytes = bytes;
   }
   d.block_id = 0;
+  d.block_path_valid = false;
   if (run(id, d.d_block) != 0) return -1;
   if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
+  if (is_path_needed) {
+    if (d.grow(d.d_block_path, d.block_path_bytes, bytes) != FAP_OK)
+      return fail(FAP_E_NOMEM, std::string("part path block: ") + fap_last_error());
+    if (run_pred(id, d.d_block, d.d_block_path) != 0) return -1;
+    d.block_path_valid = true;
+  }
   d.block_id = id;
   d.block_host_valid = false;
   current_subgraph_id = id;
   status_ = 0;
   error_.clear();
-  if (is_path_needed) error_ = "path matrices are not produced (is_path_needed ignored)";
   return 0;
 }
 
@@ -430,9 +474,17 @@ int32_t fapGraph::solveSubGraphs(const std::vector<int32_t> &ids, bool is_path_n
       if (run(uids[i], blocks[i]) != 0) return -1;
     if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
   }
+  d.multi_path_valid = false;
+  if (is_path_needed) {
+    if (d.grow(d.d_multi_path, d.multi_path_bytes, bytes) != FAP_OK)
+      return fail(FAP_E_NOMEM, std::string("part path blocks: ") + fap_last_error());
+    for (size_t i = 0; i < uids.size(); i++)
+      if (run_pred(uids[i], blocks[i], d.d_multi_path + off[uids[i]] * num_vertexs) != 0)
+        return -1;
+    d.multi_path_valid = true;
+  }
   d.multi_off = off;
   current_subgraph_id = ids.empty() ? current_subgraph_id : ids.back();
-  if (is_path_needed) error_ = "path matrices are not produced (is_path_needed ignored)";
   return 0;
 }
 
@@ -465,8 +517,29 @@ std::vector<float> fapGraph::getAllDistance() {
 }
 
 std::vector<int32_t> fapGraph::getAllPath() {
-  fail(FAP_E_STATE, "path matrices are not produced");
-  return {};
+  reset_status();
+  if (!full_solved_ || !dev_ || !dev_->all_path_valid) {
+    fail(FAP_E_STATE, "solve(true) has not produced the full path matrix");
+    return {};
+  }
+  path.resize((size_t)num_vertexs * num_vertexs);
+  if (!path.empty() && fap_memcpy_d2h(dev_->ctx, path.data(), dev_->d_all_path,
+                                      path.size() * sizeof(int32_t)) != FAP_OK) {
+    fail(FAP_E_HIP, fap_last_error());
+    return {};
+  }
+  return path;
+}
+
+const int32_t *fapGraph::subGraphPathDevice(int32_t id) {
+  if (!dev_ || !part_ok(id)) return nullptr;
+  Device &d = *dev_;
+  if (d.block_id == id) return d.block_path_valid ? d.d_block_path : nullptr;
+  auto it = d.multi_off.find(id);
+  if (it != d.multi_off.end()) return d.multi_path_valid ? d.d_multi_path + it->second * num_vertexs : nullptr;
+  if (full_solved_ && d.all_path_valid)
+    return d.d_all_path + (int64_t)C_BlockVer_offset[id] * num_vertexs;
+  return nullptr;
 }
 
 int32_t fapGraph::getCurrentSubGraphId() { return current_subgraph_id; }
@@ -519,9 +592,21 @@ std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
   return f;
 }
 
-std::vector<int32_t> fapGraph::getSubGraphPath(int32_t) {
-  fail(FAP_E_STATE, "path matrices are not produced");
-  return {};
+std::vector<int32_t> fapGraph::getSubGraphPath(int32_t id) {
+  reset_status();
+  const int32_t *p = subGraphPathDevice(id);
+  if (!p) {
+    fail(FAP_E_STATE, "part " + std::to_string(id) + " has no path rows (solve with is_path_needed)");
+    return {};
+  }
+  std::vector<int32_t> out((size_t)C_BlockVer_num[id] * num_vertexs);
+  if (!out.empty() &&
+      fap_memcpy_d2h(dev_->ctx, out.data(), p, out.size() * sizeof(int32_t)) != FAP_OK) {
+    fail(FAP_E_HIP, fap_last_error());
+    return {};
+  }
+  subgraph_path = out;
+  return out;
 }
 
 std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
@@ -549,9 +634,26 @@ std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
   return f;
 }
 
-std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t) {
-  fail(FAP_E_STATE, "path matrices are not produced");
-  return {};
+std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t v) {
+  reset_status();
+  if (v < 0 || v >= num_vertexs || !part_ok(current_subgraph_id) ||
+      graph_id[v] != current_subgraph_id) {
+    fail(FAP_E_ARG, "vertex is not in the current part");
+    return {};
+  }
+  const int32_t *p = subGraphPathDevice(current_subgraph_id);
+  if (!p) {
+    fail(FAP_E_STATE, "current part has no path rows (solve with is_path_needed)");
+    return {};
+  }
+  const int64_t row = ed2st[v] - C_BlockVer_offset[current_subgraph_id];
+  std::vector<int32_t> out(num_vertexs);
+  if (fap_memcpy_d2h(dev_->ctx, out.data(), p + row * num_vertexs,
+                     out.size() * sizeof(int32_t)) != FAP_OK) {
+    fail(FAP_E_HIP, fap_last_error());
+    return {};
+  }
+  return out;
 }
 
 float fapGraph::getDistanceP2P() {
@@ -575,8 +677,45 @@ int32_t fapGraph::getDistanceP2P(int32_t src, int32_t dst, int32_t *out) {
 }
 
 std::vector<int32_t> fapGraph::getPathP2P() {
-  fail(FAP_E_STATE, "path matrices are not produced");
+  fail(FAP_E_ARG, "getPathP2P() needs (src, dst)");
   return {};
+}
+
+std::vector<int32_t> fapGraph::getPathP2P(int32_t src, int32_t dst) {
+  reset_status();
+  if (src < 0 || src >= num_vertexs || dst < 0 || dst >= num_vertexs) {
+    fail(FAP_E_ARG, "bad vertex id");
+    return {};
+  }
+  if (!preconditioned_) {
+    fail(FAP_E_STATE, "preCondition() not run");
+    return {};
+  }
+  const int32_t part = graph_id[src];
+  const int32_t *p = subGraphPathDevice(part);
+  if (!p) {
+    fail(FAP_E_STATE, "the part of src has no path rows (solve with is_path_needed)");
+    return {};
+  }
+  const int64_t row = ed2st[src] - C_BlockVer_offset[part];
+  std::vector<int32_t> pr(num_vertexs);
+  if (fap_memcpy_d2h(dev_->ctx, pr.data(), p + row * num_vertexs, pr.size() * sizeof(int32_t)) !=
+      FAP_OK) {
+    fail(FAP_E_HIP, fap_last_error());
+    return {};
+  }
+  if (pr[dst] < 0) return {};
+  std::vector<int32_t> seq{dst};
+  for (int32_t x = dst, steps = 0; x != src; steps++) {
+    x = pr[x];
+    if (x < 0 || x >= num_vertexs || steps >= num_vertexs) {
+      fail(FAP_E_STATE, "path rows are inconsistent");
+      return {};
+    }
+    seq.push_back(x);
+  }
+  std::reverse(seq.begin(), seq.end());
+  return seq;
 }
 
 int32_t fapGraph::subGraphChecksum(int32_t id, uint64_t *sum, uint64_t *weighted_sum) {

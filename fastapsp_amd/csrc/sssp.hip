@@ -202,6 +202,249 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Grouped sources: one workgroup runs near-far for S sources at once.
+//
+// Distances live interleaved, D[v * S + s], so the S values of a vertex are
+// one 16- or 32-byte segment.  A queue entry is a vertex; S consecutive lanes
+// handle it together, lane s for source s, walking the vertex's arcs in
+// lockstep: one arc load serves all S sources and the S distance reads of a
+// neighbour are one segment.  Per vertex two masks record which sources
+// still have to be relaxed from it: `dirty` (improved since the vertex was
+// last processed) and `pend` (improved to a value at or above that source's
+// threshold while the vertex waits in the far pile).  Processing a vertex
+// takes its dirty mask (atomic exchange) and relaxes only those sources.
+// Only a far split clears pend bits (no relaxation runs beside it), so a far
+// improvement is never lost to a concurrent processing of its vertex; a pend
+// bit left over after its value was relaxed only lowers that source's next
+// threshold.  Each source keeps
+// its own threshold (a register of its lanes); when the near queue drains,
+// every source with pending far values moves its threshold to its smallest
+// pending value + delta and the far pile is split per source.
+//
+// Relaxation is label-correcting: whenever D[v][s] decreases, bit s of
+// dirty[v] is set and v is queued (near or far), so v is processed again;
+// at exit every arc satisfies D[v][s] <= D[u][s] + w, so the values are the
+// exact distances whatever order the relaxations ran in.
+template <int S, int NT>
+__global__ __launch_bounds__(NT) void sssp_group_kernel(
+    const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
+    const int32_t *__restrict__ sources, int n_src, int32_t *out, int64_t ld,
+    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int64_t d_off,
+    int delta) {
+  constexpr int NG = NT / S;  // queue entries per pass
+  constexpr uint32_t SMASK = (1u << S) - 1;
+  __shared__ uint32_t s_cnt[6];
+  __shared__ int s_min[S];
+  __shared__ int32_t *s_rows[S];
+  __shared__ uint32_t s_n0;
+  uint32_t *base = ws + (int64_t)blockIdx.x * ws_stride;
+  uint32_t *const qa0 = base, *const qb0 = base + V, *const fa0 = base + 2 * (int64_t)V,
+                  *const fb0 = base + 3 * (int64_t)V;
+  int32_t *const stamp = (int32_t *)(base + 4 * (int64_t)V);
+  int32_t *const farmark = (int32_t *)(base + 5 * (int64_t)V);
+  uint32_t *const dirty = base + 6 * (int64_t)V;
+  uint32_t *const pend = base + 7 * (int64_t)V;
+  int32_t *const D = (int32_t *)(base + d_off);  // V * S, 16-byte aligned
+  const int tid = threadIdx.x, lane = tid & 63, sl = tid % S, grp = tid / S;
+  const int gl = lane - sl;  // first lane of this lane's group
+  for (int v = tid; v < V; v += NT) {
+    stamp[v] = 0;
+    farmark[v] = -1;
+  }
+  if (tid < 6) s_cnt[tid] = 0;
+  int round_tag = 0;
+  int ci = 0;
+  const int ngroups = (n_src + S - 1) / S;
+  for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+    const int s0 = gi * S;
+    const int ns = min(S, n_src - s0);
+    const int tag = gi;
+    {
+      const int4 inf4 = make_int4(FAP_INF, FAP_INF, FAP_INF, FAP_INF);
+      int4 *D4 = reinterpret_cast<int4 *>(D);
+      for (int64_t x = tid; x < (int64_t)V * S / 4; x += NT) D4[x] = inf4;
+    }
+    for (int v = tid; v < V; v += NT) {
+      dirty[v] = 0;
+      pend[v] = 0;
+    }
+    if (tid < ns) s_rows[tid] = rowptr ? rowptr[s0 + tid] : out + (int64_t)(s0 + tid) * ld;
+    round_tag++;  // stamps of the initial queue
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t n0 = 0;
+      for (int t = 0; t < ns; t++) {
+        const int src = sources[s0 + t];
+        D[(int64_t)src * S + t] = 0;
+        dirty[src] |= 1u << t;
+        if (stamp[src] != round_tag) {
+          stamp[src] = round_tag;
+          qa0[n0++] = (uint32_t)src;
+        }
+      }
+      s_n0 = n0;
+    }
+    __syncthreads();
+    uint32_t *qa = qa0, *qb = qb0, *fa = fa0, *fb = fb0;
+    uint32_t n_near = s_n0, n_far = 0;
+    int thr = delta;  // this lane's source's threshold
+    for (;;) {
+      // ---- near rounds ----
+      while (n_near > 0) {
+        round_tag++;
+        uint32_t *cn = &s_cnt[2 * ci];
+        for (uint32_t i0 = 0; i0 < n_near; i0 += NG) {
+          const uint32_t i = i0 + grp;
+          const bool valid = i < n_near;
+          int u = 0;
+          uint32_t m = 0;
+          if (valid) {
+            u = (int)qa[i];
+            if (sl == 0) m = atomicExch(dirty + u, 0u);
+          }
+          m = (uint32_t)__shfl((int)m, gl);
+          const bool act = (m >> sl) & 1;
+          int du = 0, e = 0, e_end = 0;
+          // read at device scope, after the mask was taken: an improvement
+          // whose dirty bit is in m completed its atomicMin before that bit
+          // was set, so this sees it
+          if (act)
+            du = __hip_atomic_load(D + (int64_t)u * S + sl, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+          if (m) {
+            e = row_offset[u];
+            e_end = row_offset[u + 1];
+          }
+          while (__any(e < e_end)) {
+            bool in = false, fr = false;
+            int v = 0;
+            if (e < e_end) {
+              const int2 ed = edges[e];
+              e++;
+              v = ed.x;
+              if (act) {
+                const int nd = du + ed.y;
+                int32_t *dp = D + (int64_t)v * S + sl;
+                if (nd < *dp) {
+                  const int old = atomicMin(dp, nd);
+                  if (nd < old) {
+                    in = nd < thr;
+                    fr = !in;
+                  }
+                }
+              }
+            }
+            const uint32_t gn = (uint32_t)(__ballot(in) >> gl) & SMASK;
+            const uint32_t gf = (uint32_t)(__ballot(fr) >> gl) & SMASK;
+            bool pn = false, pf = false;
+            if (sl == 0 && (gn | gf)) {
+              atomicOr(dirty + v, gn | gf);
+              if (gf) {
+                atomicOr(pend + v, gf);
+                pf = atomicExch(farmark + v, tag) != tag;
+              }
+              if (gn) pn = atomicMax(stamp + v, round_tag) < round_tag;
+            }
+            wave_push(pn, (uint32_t)v, qb, &cn[0]);
+            wave_push(pf, (uint32_t)v, fa + n_far, &cn[1]);
+          }
+        }
+        const int cz = ci == 2 ? 0 : ci + 1;
+        if (tid == 0) {
+          s_cnt[2 * cz] = 0;
+          s_cnt[2 * cz + 1] = 0;
+        }
+        __syncthreads();
+        n_near = cn[0];
+        n_far += cn[1];
+        ci = cz;
+        uint32_t *t = qa;
+        qa = qb;
+        qb = t;
+      }
+      if (n_far == 0) break;
+      // ---- per-source smallest pending far value ----
+      if (tid < S) s_min[tid] = 0x7FFFFFFF;
+      __syncthreads();
+      int lmin = 0x7FFFFFFF;
+      for (uint32_t i = grp; i < n_far; i += NG) {
+        const int v = (int)fa[i];
+        if (farmark[v] == tag && ((pend[v] >> sl) & 1)) lmin = min(lmin, D[(int64_t)v * S + sl]);
+      }
+      for (int off = 32; off >= S; off >>= 1) lmin = min(lmin, __shfl_xor(lmin, off));
+      if (lane < S) atomicMin(&s_min[lane], lmin);
+      __syncthreads();
+      bool any = false;
+#pragma unroll
+      for (int t = 0; t < S; t++) any |= s_min[t] != 0x7FFFFFFF;
+      if (!any) break;  // every far entry was stale
+      if (s_min[sl] != 0x7FFFFFFF) thr = s_min[sl] + delta;
+      round_tag++;
+      // ---- split: pending values below the new thresholds go near ----
+      uint32_t *cn = &s_cnt[2 * ci];
+      for (uint32_t i0 = 0; i0 < n_far; i0 += NG) {
+        const uint32_t i = i0 + grp;
+        int v = 0;
+        bool live = false;
+        uint32_t p = 0;
+        if (i < n_far) {
+          v = (int)fa[i];
+          live = farmark[v] == tag;
+          if (live) p = pend[v];
+        }
+        const bool go = ((p >> sl) & 1) && D[(int64_t)v * S + sl] < thr;
+        const uint32_t gb = (uint32_t)(__ballot(go) >> gl) & SMASK;
+        bool pn = false, pf = false;
+        if (sl == 0 && live) {
+          const uint32_t rest = p & ~gb;
+          if (gb) {
+            pend[v] = rest;
+            pn = atomicMax(stamp + v, round_tag) < round_tag;
+          }
+          if (rest) pf = true;
+          else farmark[v] = -1;
+        }
+        wave_push(pn, (uint32_t)v, qa, &cn[0]);
+        wave_push(pf, (uint32_t)v, fb, &cn[1]);
+      }
+      const int cz = ci == 2 ? 0 : ci + 1;
+      if (tid == 0) {
+        s_cnt[2 * cz] = 0;
+        s_cnt[2 * cz + 1] = 0;
+      }
+      __syncthreads();
+      n_near = cn[0];
+      n_far = cn[1];
+      ci = cz;
+      uint32_t *t = fa;
+      fa = fb;
+      fb = t;
+    }
+    // ---- rows out: D[v][s] -> row s, column v ----
+    // the last values were written by atomics at the device cache: drop any
+    // older copies from this CU's vector cache before reading them plainly
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int v = tid; v < V; v += NT) {
+      int32_t d[S];
+      const int4 *src4 = reinterpret_cast<const int4 *>(D + (int64_t)v * S);
+#pragma unroll
+      for (int q = 0; q < S / 4; q++) {
+        const int4 x = src4[q];
+        d[4 * q] = x.x;
+        d[4 * q + 1] = x.y;
+        d[4 * q + 2] = x.z;
+        d[4 * q + 3] = x.w;
+      }
+#pragma unroll
+      for (int t = 0; t < S; t++)
+        if (t < ns) s_rows[t][v] = d[t];
+    }
+    __syncthreads();  // D and s_rows are rewritten by the next group
+  }
+}
+
 }  // namespace fapk
 
 namespace fapi {
@@ -212,38 +455,64 @@ namespace fapi {
 
 // Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
 // pointers) when d_rowptr is not null.
+namespace {
+
+template <int S, int NT>
+void launch_group(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t *src, int n_src,
+                  int32_t *out, int64_t ld, int32_t *const *rowptr, uint32_t *ws,
+                  int64_t stride, int64_t d_off, int delta) {
+  hipLaunchKernelGGL((fapk::sssp_group_kernel<S, NT>), dim3(grid), dim3(NT), 0, s, g->row_offset,
+                     g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride, d_off, delta);
+}
+
+template <int NT>
+void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t *src,
+                   int n_src, int32_t *out, int64_t ld, int32_t *const *rowptr, uint32_t *ws,
+                   int64_t stride, int delta) {
+  hipLaunchKernelGGL(fapk::sssp_nf_kernel<NT>, dim3(grid), dim3(NT), 0, s, g->row_offset,
+                     g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride, delta);
+}
+
+}  // namespace
+
+// Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
+// pointers) when d_rowptr is not null.  Sources are taken in groups of
+// `group` (1 = one source per workgroup pass, 4 or 8 = grouped kernel);
+// consecutive sources share a group, so callers list nearby sources
+// together (a part's boundary vertices are).
 int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
                   int32_t *d_out, int64_t ld_out, int32_t *const *d_rowptr, hipStream_t s) {
   if (n_src <= 0) return FAP_OK;
   const int V = g->V;
   if (V <= 0) return fail(FAP_E_ARG, "sssp: empty graph");
-  // per-workgroup state: 6 V words; as many workgroups as sources, up to
-  // 2 per CU and a 4 GiB state budget
-  const int64_t stride = 6 * (int64_t)V;
-  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)4 << 30) / (stride * 4));
-  int threads, delta;
-  sssp_settings(ctx, g, &threads, &delta);
+  int threads, delta, group;
+  sssp_settings(ctx, g, n_src, &threads, &delta, &group);
+  // per-workgroup state: 6 V words (single) or 8 V + S V words (grouped),
+  // rows of 64 words; as many workgroups as source groups, up to 4 / 2 / 1
+  // per CU (256 / 512 / 1024 threads) and an 8 GiB state budget
+  const int64_t vw = ((int64_t)V + 63) & ~(int64_t)63;
+  const int64_t d_off = 8 * vw;
+  const int64_t stride = group == 1 ? 6 * vw : d_off + (int64_t)group * vw;
+  const int64_t units = (n_src + group - 1) / group;
+  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)8 << 30) / (stride * 4));
   const int per_cu = threads == 256 ? 4 : threads == 512 ? 2 : 1;
   const int grid = (int)std::max<int64_t>(
-      1, std::min<int64_t>({(int64_t)n_src, (int64_t)ctx->num_cus * per_cu, by_mem}));
+      1, std::min<int64_t>({units, (int64_t)ctx->num_cus * per_cu, by_mem}));
   FAP_TRY(ensure_ws(ctx, ctx->ws_queue, (size_t)grid * stride * sizeof(uint32_t)));
   uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
-  switch (threads) {
-    case 256:
-      hipLaunchKernelGGL(fapk::sssp_nf_kernel<256>, dim3(grid), dim3(256), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
-                         delta);
-      break;
-    case 512:
-      hipLaunchKernelGGL(fapk::sssp_nf_kernel<512>, dim3(grid), dim3(512), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
-                         delta);
-      break;
-    default:
-      hipLaunchKernelGGL(fapk::sssp_nf_kernel<1024>, dim3(grid), dim3(1024), 0, s, g->row_offset,
-                         g->edges, V, d_sources, n_src, d_out, ld_out, d_rowptr, ws, stride,
-                         delta);
-      break;
+  const int32_t *src = d_sources;
+  if (group == 1) {
+    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
+    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
+    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
+  } else if (group == 4) {
+    if (threads == 256) launch_group<4, 256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
+    else if (threads == 512) launch_group<4, 512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
+    else launch_group<4, 1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
+  } else {
+    if (threads == 256) launch_group<8, 256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
+    else if (threads == 512) launch_group<8, 512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
+    else launch_group<8, 1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
   }
   FAP_CHECK_LAUNCH();
   return FAP_OK;

@@ -83,10 +83,20 @@ void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead) {
                                                    : false;
 }
 
-void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int *threads, int *delta) {
+void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
+                   int *delta, int *group) {
   int32_t p[2] = {1024, 8};
   const bool cached = tune_lookup(ctx, "sssp", fap_tune_key_sssp(g->V, g->E), p);
   *threads = (ctx && ctx->opt_sssp_threads > 0) ? ctx->opt_sssp_threads : cached ? p[0] : 1024;
+  // Grouped sources cost ~1.2x the rounds of one source but share arc and
+  // distance loads: worth it once there are enough groups to fill the GPU.
+  if (ctx && ctx->opt_sssp_group > 0) {
+    *group = ctx->opt_sssp_group;
+  } else {
+    const int per_cu = *threads == 256 ? 4 : *threads == 512 ? 2 : 1;
+    const int64_t slots = (int64_t)(ctx ? ctx->num_cus : 256) * per_cu;
+    *group = (n_src + 7) / 8 >= slots ? 8 : (n_src + 3) / 4 >= slots ? 4 : 1;
+  }
   if (ctx && ctx->opt_sssp_delta > 0) {
     *delta = ctx->opt_sssp_delta;
   } else if (g->delta > 0) {

@@ -188,6 +188,35 @@ class Graph:
         check(load().fapg_distance_p2p(self.handle, int(src), int(dst), C.byref(v)))
         return v.value
 
+    # path rows (solved with is_path_needed=True)
+    def getSubGraphPath(self, sub_graph_id: int) -> np.ndarray:
+        n = int(self.part_info()[0][sub_graph_id])
+        out = np.empty((n, self.num_vertexs), np.int32)
+        check(load().fapg_get_subgraph_path(self.handle, int(sub_graph_id), _p32(out)),
+              "getSubGraphPath")
+        return out
+
+    def getAllPath(self) -> np.ndarray:
+        V = self.num_vertexs
+        out = np.empty((V, V), np.int32)
+        check(load().fapg_get_all_path(self.handle, _p32(out)), "getAllPath")
+        return out
+
+    def getPathFromOnePoint(self, v: int) -> np.ndarray:
+        out = np.empty(self.num_vertexs, np.int32)
+        check(load().fapg_get_path_from_one_point(self.handle, int(v), _p32(out)),
+              "getPathFromOnePoint")
+        return out
+
+    def getPathP2P(self, src: int, dst: int) -> np.ndarray:
+        """Vertices src .. dst of a shortest path (empty when unreachable)."""
+        cap = self.num_vertexs
+        out = np.empty(cap, np.int32)
+        n = C.c_int32()
+        check(load().fapg_path_p2p(self.handle, int(src), int(dst), _p32(out), cap, C.byref(n)),
+              "getPathP2P")
+        return out[: n.value].copy()
+
     def checksum(self, sub_graph_id: int):
         s, w = C.c_uint64(), C.c_uint64()
         check(load().fapg_subgraph_checksum(self.handle, int(sub_graph_id), C.byref(s), C.byref(w)))

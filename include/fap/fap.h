@@ -18,8 +18,11 @@
 //   * solve() leaves getCurrentSubGraphId() at K (the reference never sets
 //     it) and getSubGraphDistance(id) then returns part id's rows of the full
 //     matrix.
-//   * path matrices (is_path_needed) are not produced yet: path getters
-//     return an empty vector and set lastError().
+//   * path matrices (is_path_needed) hold, per row, the vertex before each
+//     column on a shortest path (the reference's semantics); which of several
+//     equally short paths is not fixed (the reference's backends differ too).
+//   * getPathP2P(src, dst) returns the vertex sequence src .. dst (the
+//     no-argument reference stub, fap/fap.cpp:267-269, reports an error).
 #ifndef FAP_FAP_H_
 #define FAP_FAP_H_
 
@@ -79,6 +82,8 @@ class fapGraph {
   int32_t ensure_device();
   int32_t run(int32_t subgraph_id, int32_t *d_rowblock);
   int32_t run_batch(const std::vector<int32_t> &ids, const std::vector<int32_t *> &blocks);
+  int32_t run_pred(int32_t subgraph_id, const int32_t *d_rowblock, int32_t *d_predblock);
+  const int32_t *subGraphPathDevice(int32_t sub_garph_id);
   void init_from_csr(int32_t V, std::vector<int32_t> &&ro, std::vector<int32_t> &&col,
                      std::vector<int32_t> &&w, bool directed_graph);
   bool part_ok(int32_t id) const;
@@ -157,6 +162,10 @@ class fapGraph {
   // distance between two original vertex ids, from the resident result
   // (solve(), or solveSubGraph(part of src)); -1 if not available
   int32_t getDistanceP2P(int32_t src, int32_t dst, int32_t *out);
+  // vertex sequence src .. dst of a shortest path, from the resident path
+  // rows of src's part (solved with is_path_needed); empty if dst is not
+  // reachable (status() == 0) or on error (status() != 0)
+  std::vector<int32_t> getPathP2P(int32_t src, int32_t dst);
   // wrapping sums over a solved part block on the device (position keyed)
   int32_t subGraphChecksum(int32_t sub_garph_id, uint64_t *sum, uint64_t *weighted);
 };

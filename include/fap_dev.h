@@ -69,6 +69,8 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "fw_tile"      0/64/128 FW block size [64]
  *   "sssp_threads" 0/256/512/1024  threads per SSSP workgroup (one source each) [1024]
  *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 x mean weight]
+ *   "sssp_group"   0/1/4/8  sources per SSSP workgroup pass [8 or 4 when that still
+ *                           gives a workgroup per CU slot, else 1]
  *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
@@ -190,6 +192,22 @@ int fap_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
 int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                              int32_t *const *rowblocks, int64_t ld_rb, void *stream);
+
+/* Predecessor ("path") rows for exact distance rows -- the path matrices the
+ * reference builds when is_path_needed (fap/kernel/batch_sssp.h:82-83,110,
+ * fap/kernel/floyd.h:44, fap/kernel/minplus.h:57-60,
+ * fap/utils/util_Myalgorithm.h:50-51,71): pred[i][j] is the vertex before j
+ * on a shortest path from row i's vertex to j, the row's vertex itself at
+ * j == that vertex, -1 when j is unreachable.  d_row_vertex holds the rows'
+ * vertices (device).  Which of several equally short paths is not fixed.
+ * Builds the graph's transposed CSR on first use.  Synchronous. */
+int fap_pred_rows(fap_ctx *ctx, fap_graph_dev *g, const int32_t *d_dist, int64_t ld,
+                  const int32_t *d_row_vertex, int32_t nrows, int32_t *d_pred, int64_t ld_pred,
+                  void *stream);
+/* The same for a part's row block (rows = st2ed[start .. start + n_sub)). */
+int fap_subgraph_pred(fap_ctx *ctx, fap_graph_dev *g, int32_t start, int32_t n_sub,
+                      const int32_t *rowblock, int64_t ld_rb, int32_t *predblock, int64_t ld_pred,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,10 @@
  *   fapg_get_distance_from_one_point <- getDistanceFromOnePoint(v) fap/fap.cpp:245-252
  *   fapg_distance_p2p       <- getDistanceP2P()    fap/fap.cpp:263-265 (a stub there)
  *   fapg_check_result       <- check_result(...)   fap/fap.cpp:271-277
+ *   fapg_get_all_path       <- getAllPath()        fap/fap.cpp:218-221
+ *   fapg_get_subgraph_path  <- getSubGraphPath(id) fap/fap.cpp:240-243
+ *   fapg_get_path_from_one_point <- getPathFromOnePoint(v) fap/fap.cpp:254-261
+ *   fapg_path_p2p           <- getPathP2P()        fap/fap.cpp:267-269 (a stub there)
  */
 #ifndef FAP_GRAPH_H_
 #define FAP_GRAPH_H_
@@ -76,6 +80,15 @@ int fapg_get_all_distance_f32(fapg *g, float *out);
 /* row of vertex v in the current part (V floats) */
 int fapg_get_distance_from_one_point(fapg *g, int32_t v, float *out);
 int fapg_distance_p2p(fapg *g, int32_t src, int32_t dst, int32_t *out);
+/* path rows (solved with is_path_needed): the vertex before each column on a
+ * shortest path from the row's vertex, the vertex itself on its own column,
+ * -1 when unreachable; shapes as the distance getters */
+int fapg_get_subgraph_path(fapg *g, int32_t id, int32_t *out);
+int fapg_get_all_path(fapg *g, int32_t *out);
+int fapg_get_path_from_one_point(fapg *g, int32_t v, int32_t *out);
+/* vertices src .. dst of a shortest path into out[cap]; *len = 0 when dst is
+ * unreachable; FAP_E_ARG when cap is too small (*len says how many) */
+int fapg_path_p2p(fapg *g, int32_t src, int32_t dst, int32_t *out, int32_t cap, int32_t *len);
 /* device-side checksums of a solved part block (see fap_checksum_i32) */
 int fapg_subgraph_checksum(fapg *g, int32_t id, uint64_t *sum, uint64_t *weighted);
 /* check_ans restated (deterministic samples); *ok = 1 when all samples match */

@@ -280,4 +280,57 @@ void orc_run_part(int32_t id, int32_t V, const int32_t *row_offset,
                     n_sub, V, st2ed);
 }
 
+int64_t orc_check_pred(const int32_t *dist, int64_t ld, const int32_t *pred, int64_t ldp,
+                       const int32_t *own, int32_t nrows, int32_t V, const int32_t *row_offset,
+                       const int32_t *col, const int32_t *w) {
+  // in-arcs of every vertex: {source, weight}
+  std::vector<int32_t> off(V + 1, 0);
+  for (int32_t u = 0; u < V; u++)
+    for (int32_t e = row_offset[u]; e < row_offset[u + 1]; e++) off[col[e] + 1]++;
+  for (int32_t v = 0; v < V; v++) off[v + 1] += off[v];
+  std::vector<int32_t> fill(off.begin(), off.end() - 1), isrc(off[V]), iw(off[V]);
+  for (int32_t u = 0; u < V; u++)
+    for (int32_t e = row_offset[u]; e < row_offset[u + 1]; e++) {
+      isrc[fill[col[e]]] = u;
+      iw[fill[col[e]]++] = w[e];
+    }
+  int64_t bad = 0;
+  std::vector<int8_t> state(V);
+  std::vector<int32_t> stack;
+  for (int32_t i = 0; i < nrows; i++) {
+    const int32_t *d = dist + (int64_t)i * ld;
+    const int32_t *p = pred + (int64_t)i * ldp;
+    const int32_t me = own[i];
+    std::vector<int8_t> ok(V, 0);
+    for (int32_t j = 0; j < V; j++) {
+      if (j == me) {
+        ok[j] = p[j] == me;
+      } else if (d[j] >= ORC_INF) {
+        ok[j] = p[j] == -1;
+      } else if (p[j] >= 0 && p[j] < V && d[p[j]] < ORC_INF) {
+        for (int32_t e = off[j]; e < off[j + 1] && !ok[j]; e++)
+          ok[j] = isrc[e] == p[j] && d[p[j]] + iw[e] == d[j];
+      }
+      if (!ok[j]) bad++;
+    }
+    // walks: 0 unvisited, 1 on the current walk, 2 reaches me
+    std::fill(state.begin(), state.end(), 0);
+    if (me >= 0 && me < V) state[me] = 2;
+    for (int32_t j = 0; j < V; j++) {
+      if (state[j] || d[j] >= ORC_INF || !ok[j]) continue;
+      stack.clear();
+      int32_t x = j;
+      while (x >= 0 && x < V && state[x] == 0 && ok[x] && d[x] < ORC_INF) {
+        state[x] = 1;
+        stack.push_back(x);
+        x = p[x];
+      }
+      const bool reaches = x >= 0 && x < V && state[x] == 2;
+      for (int32_t y : stack) state[y] = reaches ? 2 : 3;
+      if (!reaches) bad += (int64_t)stack.size();
+    }
+  }
+  return bad;
+}
+
 }  // extern "C"

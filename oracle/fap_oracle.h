@@ -104,6 +104,19 @@ void orc_run_part(int32_t id, int32_t V, const int32_t *row_offset,
                   const int32_t *ver_off, const int32_t *bdy_num,
                   int32_t *rowblock, int32_t *rowblock_path, int nthreads);
 
+/* Validity of predecessor ("path") rows, the semantics of the reference's
+ * path matrices (fap/kernel/batch_sssp.h:82-83,110; fap/kernel/floyd.h:44;
+ * fap/kernel/minplus.h:57-60; fap/utils/util_Myalgorithm.h:50-51,71):
+ * pred[i][own[i]] == own[i]; pred[i][j] == -1 iff dist[i][j] == INF;
+ * otherwise an arc pred -> j with dist[i][pred] + w == dist[i][j] exists and
+ * following predecessors from j reaches own[i] without a cycle.  Returns the
+ * number of cells that break a rule (0 = valid).  Not a restatement: a
+ * checker, since the reference's choice among equal paths differs between
+ * its own backends. */
+int64_t orc_check_pred(const int32_t *dist, int64_t ld, const int32_t *pred, int64_t ldp,
+                       const int32_t *own, int32_t nrows, int32_t V, const int32_t *row_offset,
+                       const int32_t *col, const int32_t *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,14 +152,27 @@ def submat_decode(sub, rowblock, start, st2ed):
     return rb
 
 
-def run_part(pid, ro, col, w, graph_id, pc, nthreads=0):
+def run_part(pid, ro, col, w, graph_id, pc, nthreads=0, with_path=False):
     V = len(ro) - 1
     n = int(pc["ver_num"][pid])
     out = np.empty((n, V), np.int32)
+    path = np.empty((n, V), np.int32) if with_path else None
     oracle().orc_run_part(_i(pid), _i(V), _p(i32(ro)), _p(i32(col)), _p(i32(w)),
                           _p(i32(graph_id)), _p(pc["st2ed"]), _p(pc["ed2st"]), _p(pc["ver_num"]),
-                          _p(pc["ver_off"]), _p(pc["bdy_num"]), _p(out), None, C.c_int(nthreads))
-    return out
+                          _p(pc["ver_off"]), _p(pc["bdy_num"]), _p(out), _p(path),
+                          C.c_int(nthreads))
+    return (out, path) if with_path else out
+
+
+def check_pred(dist, pred, own, ro, col, w) -> int:
+    """Number of cells of predecessor rows `pred` that break the reference's
+    path semantics for distance rows `dist` (0 = valid)."""
+    D, P = i32(dist), i32(pred)
+    o = i32(own)
+    f = oracle().orc_check_pred
+    f.restype = C.c_int64
+    return int(f(_p(D), _i64(D.shape[1]), _p(P), _i64(P.shape[1]), _p(o), _i(len(o)),
+                 _i(len(ro) - 1), _p(i32(ro)), _p(i32(col)), _p(i32(w))))
 
 
 # ---------------- reference CPU code (float) ----------------
@@ -234,7 +247,7 @@ def ref_precondition(K, graph_id, ro, col):
     return out
 
 
-def ref_run_part(pid, ro, col, w, graph_id, pc):
+def ref_run_part(pid, ro, col, w, graph_id, pc, with_path=False):
     V = len(ro) - 1
     ro = i32(ro)
     adj = np.diff(ro).astype(np.int32)
@@ -245,4 +258,4 @@ def ref_run_part(pid, ro, col, w, graph_id, pc):
                        _f(to_ref(w)), _p(i32(graph_id).copy()), _p(pc["st2ed"].copy()),
                        _p(pc["ed2st"].copy()), _p(pc["ver_num"].copy()), _p(pc["ver_off"].copy()),
                        _p(pc["bdy_num"].copy()), _f(out), _p(outp))
-    return out
+    return (out, outp) if with_path else out

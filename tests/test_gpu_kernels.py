@@ -173,16 +173,23 @@ def test_sssp_golden_all_sources(gpu_ctx, name):
 
 @pytest.fixture
 def sssp_config(gpu_ctx, request):
-    threads, delta = request.param
+    threads, delta, group = request.param
     gpu_ctx.set_option("sssp_threads", threads)
     gpu_ctx.set_option("sssp_delta", delta)
+    gpu_ctx.set_option("sssp_group", group)
     yield request.param
     gpu_ctx.set_option("sssp_threads", 0)
     gpu_ctx.set_option("sssp_delta", 0)
+    gpu_ctx.set_option("sssp_group", 0)
 
 
-@pytest.mark.parametrize("sssp_config", [(1024, 0), (256, 1), (512, 7), (1024, 100000)],
-                         indirect=True, ids=lambda c: f"t{c[0]}-d{c[1]}")
+SSSP_CONFIGS = [(1024, 0, 1), (256, 1, 1), (512, 7, 1), (1024, 100000, 1),
+                (1024, 0, 8), (256, 1, 8), (512, 7, 4), (1024, 100000, 4), (256, 0, 4),
+                (512, 0, 8)]
+
+
+@pytest.mark.parametrize("sssp_config", SSSP_CONFIGS, indirect=True,
+                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
 def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     u, v, w = synth.grid_graph(64, 64, 0.05, 0.02, 1, 60, 1)
     V = 4096
@@ -199,7 +206,28 @@ def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     dg.close()
 
 
-def test_sssp_directed_zero_weights(gpu_ctx):
+@pytest.mark.parametrize("sssp_config", [(1024, 0, 8), (256, 3, 4)], indirect=True,
+                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
+@pytest.mark.parametrize("name", ["grid8.npz", "rgg100.npz"])
+def test_sssp_grouped_golden(gpu_ctx, sssp_config, name):
+    """Grouped sources (several per workgroup pass), against the golden rows;
+    the source list has a ragged last group and repeats inside a group."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name))
+    ro, col, w = g["row_offset"], g["col"], g["wt"]
+    V = len(ro) - 1
+    src = np.concatenate([np.arange(V), [0, 0, V - 1]]).astype(np.int32)
+    dg = gpu_ctx.upload_graph(ro, col, w)
+    dsrc = gpu_ctx.to_device(src)
+    out = gpu_ctx.alloc((len(src), V))
+    dg.sssp(dsrc.ptr, len(src), out.ptr, V)
+    np.testing.assert_array_equal(out.get(), g["apsp"][src])
+    dg.close()
+
+
+@pytest.mark.parametrize("sssp_config", [(1024, 0, 1), (512, 0, 8)], indirect=True,
+                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
+def test_sssp_directed_zero_weights(gpu_ctx, sssp_config):
     au, av, aw = synth.rgg_graph(500, 5, 0, 9, 5)
     ro, col, w = synth.to_csr(500, au, av, aw, True)
     dg = gpu_ctx.upload_graph(ro, col, w)
@@ -269,9 +297,11 @@ def test_subgraph_solve_matches_oracle(gpu_ctx, R, K):
     dg.close()
 
 
+@pytest.mark.parametrize("sssp_config", [(1024, 0, 0), (512, 0, 8)], indirect=True,
+                         ids=lambda c: f"g{c[2]}")
 @pytest.mark.parametrize("R,K,pick", [(20, 4, None), (33, 7, (7, 2, 5)), (16, 1, None),
                                        (40, 12, (3, 3, 12, 1))])
-def test_subgraph_solve_batch_matches_per_part(gpu_ctx, R, K, pick):
+def test_subgraph_solve_batch_matches_per_part(gpu_ctx, sssp_config, R, K, pick):
     """All parts (or a subset, duplicates included) in one batched call: every
     row block equals the oracle's per-part pipeline, and a guard column past
     each block's V columns is left alone."""

@@ -280,3 +280,85 @@ def test_oracle_run_part_vs_reference():
     for pid in (1, 3):
         np.testing.assert_array_equal(O.run_part(pid, ro, col, wt, gid, pc),
                                       O.from_ref(O.ref_run_part(pid, ro, col, wt, gid, pc)))
+
+
+# ---------------- predecessor rows (path matrices) ----------------
+
+def _pred_graph(seed, zero=False):
+    from fastapsp_amd import synth
+    au, av, aw = synth.rgg_graph(200, 5, 0 if zero else 1, 9, seed)
+    ro, col, w = synth.to_csr(200, au, av, aw, True)
+    return ro, col, w
+
+
+@pytest.mark.parametrize("zero", [False, True])
+def test_pred_checker_accepts_dijkstra_parents(zero):
+    """The checker passes the oracle's Dijkstra parent rows (the reference's
+    dijkstra_path semantics, fap/kernel/batch_sssp.h:82-83,110), zero-weight
+    arcs included."""
+    ro, col, w = _pred_graph(3, zero)
+    own = np.arange(0, 200, 7, dtype=np.int32)
+    D, P = [], []
+    for s in own:
+        d, p = O.dijkstra(int(s), ro, col, w, with_path=True)
+        D.append(d)
+        P.append(p)
+    D, P = np.array(D), np.array(P)
+    assert O.check_pred(D, P, own, ro, col, w) == 0
+    if O.ref_available():
+        for i, s in enumerate(own[:5]):
+            rd, rp = O.ref_dijkstra(int(s), ro, col, w)
+            np.testing.assert_array_equal(O.from_ref(rd), D[i])
+            assert O.check_pred(D[i:i + 1], rp[None, :], own[i:i + 1], ro, col, w) == 0
+
+
+def test_pred_checker_accepts_pipeline_paths():
+    """Path rows of the oracle's whole per-part pipeline (Dijkstra, FW and
+    min-plus path updates) and, where built, of the reference's own run()."""
+    from fastapsp_amd import synth
+    u, v, ww = synth.grid_graph(20, 20, 0.1, 0.03, 1, 30, 4)
+    V = 400
+    ro, col, w = synth.to_csr(V, u, v, ww, False)
+    gid = (np.arange(V) * 4 // V + 1).astype(np.int32)
+    pc = O.precondition(4, gid, ro, col)
+    for pid in (1, 3):
+        d, p = O.run_part(pid, ro, col, w, gid, pc, with_path=True)
+        s = int(pc["ver_off"][pid])
+        own = pc["st2ed"][s:s + int(pc["ver_num"][pid])]
+        assert O.check_pred(d, p, own, ro, col, w) == 0
+        if O.ref_available():
+            rd, rp = O.ref_run_part(pid, ro, col, w, gid, pc, with_path=True)
+            assert O.check_pred(O.from_ref(rd), rp, own, ro, col, w) == 0
+
+
+def test_pred_checker_flags_planted_mistakes():
+    ro, col, w = _pred_graph(5, zero=True)
+    own = np.array([0, 11], np.int32)
+    D = np.array([O.dijkstra(int(s), ro, col, w) for s in own])
+    P = np.array([O.dijkstra(int(s), ro, col, w, with_path=True)[1] for s in own])
+    assert O.check_pred(D, P, own, ro, col, w) == 0
+    reach = np.flatnonzero((D[0] < O.INF) & (np.arange(200) != own[0]))
+    j = int(reach[len(reach) // 2])
+    bad = P.copy()
+    bad[0, own[0]] = -1                                   # own vertex not itself
+    assert O.check_pred(D, bad, own, ro, col, w) > 0
+    bad = P.copy()
+    bad[0, j] = -1                                        # reachable marked unreachable
+    assert O.check_pred(D, bad, own, ro, col, w) > 0
+    bad = P.copy()
+    nontight = [x for x in range(200) if x != P[0, j] and x != j]
+    bad[0, j] = nontight[0]                               # not a tight in-neighbour
+    assert O.check_pred(D, bad, own, ro, col, w) > 0
+    unreach = np.flatnonzero(D[0] >= O.INF)
+    if unreach.size:
+        bad = P.copy()
+        bad[0, unreach[0]] = own[0]                       # unreachable given a predecessor
+        assert O.check_pred(D, bad, own, ro, col, w) > 0
+    # a cycle through a zero-weight plateau: 0 -> 1, 0 -> 2 (w 5), 1 <-> 2 (w 0)
+    ro2 = np.array([0, 2, 3, 4], np.int32)
+    col2 = np.array([1, 2, 2, 1], np.int32)
+    w2 = np.array([5, 5, 0, 0], np.int32)
+    D2 = np.array([[0, 5, 5]], np.int32)
+    assert O.check_pred(D2, np.array([[0, 0, 0]]), [0], ro2, col2, w2) == 0
+    assert O.check_pred(D2, np.array([[0, 2, 0]]), [0], ro2, col2, w2) == 0
+    assert O.check_pred(D2, np.array([[0, 2, 1]]), [0], ro2, col2, w2) > 0
