@@ -473,8 +473,10 @@ def bench_serve(args):
     query batches against a resident 512x512 road-like grid, admitted by an
     HBM budget.  First a burst measures the saturation rate; then the same
     request mix arrives at a fixed rate (--rate-frac of saturation) on the
-    continuous-batching server and on a plain one (one lane, one request per
-    launch sequence).  Results of both runs must be identical."""
+    continuous-batching server (queries through the index over
+    --query-parts parts) and on a plain one (one lane, one request per
+    launch sequence, an SSSP per query batch).  Results of both runs must
+    be identical."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
     R = args.grid
@@ -485,8 +487,13 @@ def bench_serve(args):
         f"resident V={R * R}")
     budget = int(args.hbm_gib * (1 << 30))
     out = {}
-    with fa.Server(hbm_budget=budget, lanes=args.lanes) as srv:
+    with fa.Server(hbm_budget=budget, lanes=args.lanes, query_parts=args.query_parts) as srv:
+        t0 = time.perf_counter()
         srv.attach_graph(ro, col, wt)
+        attach_s = time.perf_counter() - t0
+        ist = srv.stats()
+        log(f"  attach {attach_s:.2f} s (index {ist['index_bytes'] / 2**30:.1f} GiB, "
+            f"{ist['index_build_ms']:.0f} ms)")
         _serve_run(srv, pool[:16], None)  # warm-up
         recs, wall = _serve_run(srv, pool, None)
         sat = _serve_summary(recs, wall, pool)
@@ -519,11 +526,82 @@ def bench_serve(args):
                    "arrival_rate_req_s": round(lam, 2), "rate_frac_of_saturation": args.rate_frac},
         "saturation": sat, "served": served, "plain_same_rate": plain,
         "served_over_plain_p99": round(plain["p99_ms"] / served["p99_ms"], 2),
-        "identical": bool(same), "server_stats": {k: st[k] for k in ("waves", "max_wave",
-                                                                     "arena_peak")},
+        "identical": bool(same),
+        "server_stats": {k: st[k] for k in ("waves", "max_wave", "arena_peak", "index_bytes",
+                                            "index_build_ms")},
+        "query_index": {"parts": args.query_parts, "attach_s": round(attach_s, 3)},
     }
     print(json.dumps(out), flush=True)
     return 0 if same else 1
+
+
+def bench_c1(args):
+    """Plain method, small: 64x64 grid (5% of edges removed), unit weights,
+    K=8, the full 4096 x 4096 matrix through the drop-in fapGraph::solve(),
+    plain (parts one after another) and batched; the reference's own CPU
+    pipeline (oracle/_ref, Dijkstra over OpenMP threads, serial FW and
+    min-plus) timed on all 8 parts as the baseline."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    R, K = 64, 8
+    u, v, w = synth.grid_graph(R, R, 0.05, 0.0, 1, 1, 1)
+    V = R * R
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    res = {}
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), K) as g:
+        g.preCondition()
+        vn, vo, bn, _ = g.part_info()
+        ops = sum(2.0 * int(vn[p]) ** 3 + 2.0 * (int(vn[p]) - int(bn[p])) * int(bn[p]) * V
+                  for p in range(1, K + 1))
+        for mode in ("plain", "batched"):
+            g.setBatched(mode == "batched")
+            g.solve()
+            ts = []
+            for _ in range(max(args.steps, 1)):
+                t0 = time.perf_counter()
+                g.solve()
+                ts.append(time.perf_counter() - t0)
+            res[mode] = min(ts)
+            res[mode + "_sum"] = sum(g.checksum(p)[1] for p in range(1, K + 1)) % (1 << 64)
+            log(f"[c1] {mode}: {1e3 * res[mode]:.2f} ms")
+        gid = g.getGraphId()
+        pc = {"st2ed": g.getMapping().astype(np.int32), "ver_num": vn.astype(np.int32),
+              "ver_off": vo.astype(np.int32), "bdy_num": bn.astype(np.int32)}
+        pc["ed2st"] = np.empty(V, np.int32)
+        pc["ed2st"][pc["st2ed"]] = np.arange(V, dtype=np.int32)
+        D = g.getAllDistanceInt32()
+    cpu = None
+    if args.cpu_rows > 0:
+        t0 = time.perf_counter()
+        rows = []
+        kind = "reference" if O.ref_available() else "port"
+        for p in range(1, K + 1):
+            if kind == "reference":
+                rows.append(O.from_ref(O.ref_run_part(p, ro, col, wt, gid, pc)))
+            else:
+                rows.append(O.run_part(p, ro, col, wt, gid, pc))
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(np.concatenate(rows), D))
+        cpu = {"value": round(ops / dt / 1e9, 3), "unit": "GOp/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())) if kind == "reference" else 1,
+               "kind": kind,
+               "sample": f"all {K} parts through the reference's CPU run() ({dt:.2f} s); "
+                         f"rows identical to the GPU's: {same}"}
+    out = {
+        "metric": "min-plus GOp/s", "value": round(ops / res["batched"] / 1e9, 1), "unit": "GOp/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * res["batched"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "64x64 grid (-5% edges), unit weights, K=8, full 4096^2 matrix "
+                               "via fapGraph::solve()", "V": V, "K": K, "ops": ops},
+        "plain_ms": round(1e3 * res["plain"], 3), "batched_ms": round(1e3 * res["batched"], 3),
+        "batched_over_plain": round(res["plain"] / res["batched"], 2),
+        "identical": res["plain_sum"] == res["batched_sum"], "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def _time_ms(lib, ctx, fn, reps=2):
@@ -714,8 +792,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-rows", type=int, default=24)
     ap.add_argument("--workload", default="dense",
-                    choices=["dense", "copy", "grid", "gridbatch", "fwsweep", "ssspsweep",
-                             "batch", "serve"])
+                    choices=["dense", "copy", "c1", "grid", "gridbatch", "fwsweep",
+                             "ssspsweep", "batch", "serve"])
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--sizes", default="256,512,1024,2048,4096,8192")
     ap.add_argument("--grid", type=int, default=512)
@@ -726,6 +804,8 @@ def main():
     ap.add_argument("--rate-frac", type=float, default=0.5)
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--hbm-gib", type=float, default=16.0)
+    ap.add_argument("--query-parts", type=int, default=256,
+                    help="parts of the served resident graph's query index (0: SSSP per query)")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: loaded if present, missing buckets tuned and saved")
     args = ap.parse_args()
@@ -739,6 +819,8 @@ def main():
         return bench_grid_pipeline(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "gridbatch":
         return bench_grid_batch(args)
     if args.workload == "batch":

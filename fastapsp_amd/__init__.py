@@ -34,7 +34,8 @@ _pu64 = C.POINTER(C.c_uint64)
 class ServeOpts(C.Structure):
     """fap_serve_opts (include/fap_serve.h)."""
     _fields_ = [("hbm_budget", C.c_int64), ("stage_bytes", C.c_int64), ("lanes", C.c_int32),
-                ("max_batch", C.c_int32), ("max_sources", C.c_int32), ("poll_us", C.c_int32)]
+                ("max_batch", C.c_int32), ("max_sources", C.c_int32), ("poll_us", C.c_int32),
+                ("query_parts", C.c_int32)]
 
 
 class ServeInfo(C.Structure):
@@ -49,7 +50,8 @@ class ServeStats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "pending", "running",
                                          "waves", "arena_bytes", "arena_used", "arena_peak",
                                          "fw_ops", "sssp_sources")] + \
-               [("max_wave", C.c_int32), ("lanes", C.c_int32)]
+               [("max_wave", C.c_int32), ("lanes", C.c_int32), ("index_bytes", C.c_int64),
+                ("index_build_ms", C.c_double)]
 
 
 # name -> (restype, argtypes)

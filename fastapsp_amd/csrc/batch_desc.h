@@ -74,6 +74,18 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
 int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources, int32_t n_src,
                   int32_t *d_out, int64_t ld_out, int32_t *const *d_rowptr, hipStream_t s);
 
+// Device bytes of SSSP state run_sssp_rows needs for n_src sources.
+size_t sssp_ws_bytes(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src);
+
+// Part matrix gather (submat.hip) and batched-closure plans (batch.hip).
+int launch_gather(const fap_graph_dev *g, int32_t start, int32_t n_sub, int32_t n_bdy,
+                  const int32_t *rowblock, int64_t ld_rb, int32_t *sub, int64_t ld_sub,
+                  hipStream_t s);
+int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int32_t *n,
+                   const int64_t *ld, int use_graph, fap_fw_plan **out);
+int fw_plan_run(fap_fw_plan *p, hipStream_t s);
+void fw_plan_destroy(fap_fw_plan *p);
+
 // Dense build of jobs already on the device; `cells` = sum n^2, `rows` = sum n.
 int launch_dense_jobs(fap_ctx *ctx, const fapk::DenseJob *d_jobs, int32_t njobs, int64_t cells,
                       int64_t rows, hipStream_t s);

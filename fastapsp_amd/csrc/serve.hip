@@ -29,6 +29,7 @@
 #include "batch_desc.h"
 #include "common.h"
 #include "fap_serve.h"
+#include "host.h"
 
 namespace fapk {
 
@@ -39,6 +40,39 @@ __global__ void answers_kernel(int32_t *const *__restrict__ rowptr,
                                int nq) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < nq) ans[q] = rowptr[qrow[q]][qdst[q]];
+}
+
+// Point queries from the index: the cell (s, t) of s's part block, as the
+// reference pipeline would produce it (fap/fap.cpp:92-181).  pinfo[p] =
+// {part offset in st2ed order, size, boundary count, first boundary row in
+// B}; M holds each part's closed matrix (row stride = size rounded up to 4)
+// at moff[p]; B the SSSP rows of all boundary vertices.
+__global__ void indexed_answers_kernel(const int32_t *__restrict__ gid,
+                                       const int32_t *__restrict__ ed2st,
+                                       const int4 *__restrict__ pinfo,
+                                       const int64_t *__restrict__ moff,
+                                       const int32_t *__restrict__ M, const int32_t *__restrict__ B,
+                                       int64_t V, const int32_t *__restrict__ qsrc,
+                                       const int32_t *__restrict__ qdst, int32_t *__restrict__ ans,
+                                       int nq) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int s = qsrc[q], t = qdst[q];
+  const int p = gid[s];
+  const int4 pi = pinfo[p];
+  const int i = ed2st[s] - pi.x;
+  const int64_t ld = ((int64_t)pi.y + 3) & ~(int64_t)3;
+  const int32_t *row = M + moff[p] + (int64_t)i * ld;
+  int r;
+  if (gid[t] == p) {
+    r = row[ed2st[t] - pi.x];
+  } else if (i < pi.z) {
+    r = B[((int64_t)pi.w + i) * V + t];
+  } else {
+    r = 0x3FFFFFFF;
+    for (int k = 0; k < pi.z; k++) r = min(r, row[k] + B[((int64_t)pi.w + k) * V + t]);
+  }
+  ans[q] = r;
 }
 
 // fap_checksum_i32 of an n x n matrix stored with row stride ld, indexed as
@@ -159,21 +193,44 @@ struct Request {
   int readers = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
 
-  size_t stage_need() const {
+  size_t stage_need(bool indexed) const {
     if (kind == FAP_REQ_GRAPH)
       return align_up(4 * ((size_t)V + 1)) + align_up(8 * (size_t)E) + sizeof(fapk::DenseJob) +
              sizeof(fapk::BatchJob) + 3 * sizeof(fapk::BatchSeg) * (size_t)((V + 63) / 64 + 1);
-    const size_t q = dst.size(), u = uniq.size();
+    const size_t q = dst.size(), u = indexed ? 0 : uniq.size();
     return 4 * u + 8 * u + 12 * q;
   }
-  size_t arena_need(int32_t resident_V) const {
+  size_t arena_need(int32_t resident_V, bool indexed) const {
     if (kind == FAP_REQ_GRAPH) return 4 * (size_t)V * (size_t)ld;
-    return 4 * uniq.size() * (size_t)resident_V;
+    return indexed ? 0 : 4 * uniq.size() * (size_t)resident_V;
   }
 };
 
 // Slack for the alignment of a wave's descriptor arrays in staging.
 constexpr size_t kStageSlack = 8 * kAlign;
+
+// Query index over the resident graph (fap_serve_attach_graph with
+// query_parts > 0).
+struct QueryIndex {
+  bool ready = false;
+  int4 *pinfo = nullptr;   // K+1: {offset, size, boundary count, first B row}
+  int64_t *moff = nullptr; // K+1: offset of each part matrix in M
+  int32_t *M = nullptr;    // closed part matrices
+  int32_t *B = nullptr;    // boundary SSSP rows (sum of boundary counts x V)
+  int64_t bytes = 0;
+  double build_ms = 0;
+  void release() {
+    void *bufs[] = {pinfo, moff, M, B};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+    pinfo = nullptr;
+    moff = nullptr;
+    M = nullptr;
+    B = nullptr;
+    bytes = 0;
+    ready = false;
+  }
+};
 
 struct Lane {
   fap_ctx *ctx = nullptr;
@@ -196,6 +253,7 @@ struct fap_server {
   char *arena_base = nullptr;
   fapi::Arena arena;
   fap_graph_dev *resident = nullptr;
+  fapi::QueryIndex index;
   // client-side device work (rows, checksums) runs on its own stream
   hipStream_t client = nullptr;
   unsigned long long *d_acc = nullptr;
@@ -262,8 +320,9 @@ bool select_wave(fap_server *S, Lane &L) {
       changed = true;
       continue;
     }
-    const size_t sn = r->stage_need();
-    const int64_t un = kind == FAP_REQ_QUERY ? (int64_t)r->uniq.size() : 0;
+    const bool indexed = S->index.ready;
+    const size_t sn = r->stage_need(indexed);
+    const int64_t un = kind == FAP_REQ_QUERY && !indexed ? (int64_t)r->uniq.size() : 0;
     if (sn > stage_cap || un > S->opts.max_sources) {
       finish(S, r, FAP_REQ_FAILED, FAP_E_NOMEM,
              "request larger than one wave (staging bytes or sources per wave)");
@@ -272,17 +331,19 @@ bool select_wave(fap_server *S, Lane &L) {
       continue;
     }
     if (stage + sn > stage_cap || sources + un > S->opts.max_sources) break;
-    const size_t an = r->arena_need(rV);
+    const size_t an = r->arena_need(rV, indexed);
     if (an > S->arena.size()) {
       finish(S, r, FAP_REQ_FAILED, FAP_E_NOMEM, "request larger than the HBM budget");
       it = S->pending.erase(it);
       changed = true;
       continue;
     }
-    const int64_t off = S->arena.alloc(an);
-    if (off < 0) break;  // wait for memory; later requests of this kind wait too
-    if (kind == FAP_REQ_GRAPH) r->block = off;
-    else r->rows = off;
+    if (an > 0) {
+      const int64_t off = S->arena.alloc(an);
+      if (off < 0) break;  // wait for memory; later requests of this kind wait too
+      if (kind == FAP_REQ_GRAPH) r->block = off;
+      else r->rows = off;
+    }
     stage += sn;
     sources += un;
     r->state = FAP_REQ_RUNNING;
@@ -349,6 +410,30 @@ int launch_wave(fap_server *S, Lane &L) {
                               s));
     FAP_TRY(launch_fw_batch(L.ctx, fd, (const fapk::BatchJob *)(L.d_stage + o_bj),
                             (const fapk::BatchSeg *)(L.d_stage + o_sg), s));
+  } else if (S->index.ready) {
+    std::vector<int32_t> qs, qd;
+    for (Request *r : L.wave)
+      for (size_t q = 0; q < r->dst.size(); q++) {
+        qs.push_back(r->uniq[r->qrow[q]]);
+        qd.push_back(r->dst[q]);
+      }
+    const size_t o_qs = put(qs.data(), 4 * qs.size());
+    const size_t o_qd = put(qd.data(), 4 * qd.size());
+    const size_t staged = off;
+    L.ans_off = off;
+    off = align_up(off + 4 * qd.size());
+    if (off > (size_t)S->opts.stage_bytes) return fail(FAP_E_NOMEM, "wave staging overflow");
+    FAP_HIP(hipMemcpyAsync(L.d_stage, h, staged, hipMemcpyHostToDevice, s));
+    const int nq = (int)qd.size();
+    const fap_graph_dev *g = S->resident;
+    hipLaunchKernelGGL(fapk::indexed_answers_kernel, dim3((nq + 63) / 64), dim3(64), 0, s,
+                       g->graph_id, g->ed2st, S->index.pinfo, S->index.moff, S->index.M,
+                       S->index.B, (int64_t)g->V, (const int32_t *)(L.d_stage + o_qs),
+                       (const int32_t *)(L.d_stage + o_qd), (int32_t *)(L.d_stage + L.ans_off),
+                       nq);
+    FAP_CHECK_LAUNCH();
+    FAP_HIP(hipMemcpyAsync(h + L.ans_off, L.d_stage + L.ans_off, 4 * (size_t)nq,
+                           hipMemcpyDeviceToHost, s));
   } else {
     const int32_t V = S->resident->V;
     std::vector<int32_t> src, qrow, qdst;
@@ -396,7 +481,7 @@ void complete_wave(fap_server *S, Lane &L) {
     if (r->kind == FAP_REQ_QUERY) {
       r->answers.assign(ans, ans + r->dst.size());
       ans += r->dst.size();
-      S->st.sssp_sources += (int64_t)r->uniq.size();
+      if (!S->index.ready) S->st.sssp_sources += (int64_t)r->uniq.size();
     } else {
       S->st.fw_ops += 2 * (int64_t)r->V * r->V * r->V;
     }
@@ -525,6 +610,86 @@ void unpin(fap_server *S, Request *r) {
   r->readers--;
 }
 
+// The query index over the resident graph g (uploaded, same CSR as `csr`):
+// partition into K parts, preCondition, SSSP rows from every boundary
+// vertex, every part's matrix gathered and closed (one batched closure).
+int build_index(fap_server *S, const HostCSR &csr, int32_t K) {
+  fap_ctx *ctx = S->lanes[0].ctx;
+  fap_graph_dev *g = S->resident;
+  hipStream_t s = ctx->stream;
+  QueryIndex &ix = S->index;
+  const auto t0 = Clock::now();
+  std::vector<int32_t> gid;
+  bisection_partition(csr, K, &gid);
+  PartInfo pi;
+  precondition(csr, K, gid, &pi);
+  FAP_TRY(fap_graph_set_partition(ctx, g, gid.data(), pi.st2ed.data(), pi.ed2st.data()));
+  const int64_t V = csr.V;
+  std::vector<int4> pinfo(K + 1, make_int4(0, 0, 0, 0));
+  std::vector<int64_t> moff(K + 1, 0);
+  std::vector<int32_t> srcs;
+  int64_t nb = 0, mwords = 0;
+  for (int32_t p = 1; p <= K; p++) {
+    const int32_t off = pi.ver_off[p], n = pi.ver_num[p], b = pi.bdy_num[p];
+    pinfo[p] = make_int4(off, n, b, (int32_t)nb);
+    moff[p] = mwords;
+    mwords += (int64_t)n * (((int64_t)n + 3) & ~(int64_t)3);
+    for (int32_t k = 0; k < b; k++) srcs.push_back(pi.st2ed[off + k]);
+    nb += b;
+  }
+  if (nb > INT32_MAX) return fail(FAP_E_ARG, "query index: too many boundary vertices");
+  const size_t b_bytes = (size_t)std::max<int64_t>(nb, 1) * V * 4;
+  const size_t m_bytes = (size_t)std::max<int64_t>(mwords, 1) * 4;
+  hipError_t e = hipMalloc((void **)&ix.B, b_bytes);
+  if (e == hipSuccess) e = hipMalloc((void **)&ix.M, m_bytes);
+  if (e == hipSuccess) e = hipMalloc((void **)&ix.pinfo, sizeof(int4) * (K + 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&ix.moff, sizeof(int64_t) * (K + 1));
+  int32_t *d_src = nullptr;
+  if (e == hipSuccess) e = hipMalloc((void **)&d_src, sizeof(int32_t) * std::max<int64_t>(nb, 1));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (d_src) (void)hipFree(d_src);
+    ix.release();
+    return fail(FAP_E_NOMEM, std::string("query index allocation: ") + hipGetErrorString(e));
+  }
+  ix.bytes = (int64_t)(b_bytes + m_bytes);
+  int rc = FAP_OK;
+  e = hipMemcpyAsync(ix.pinfo, pinfo.data(), sizeof(int4) * (K + 1), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ix.moff, moff.data(), sizeof(int64_t) * (K + 1), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && nb)
+    e = hipMemcpyAsync(d_src, srcs.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) rc = hip_fail(e, "query index upload", __FILE__, __LINE__);
+  // stage 1: every boundary vertex's row
+  if (rc == FAP_OK && nb) rc = run_sssp_rows(ctx, g, d_src, (int32_t)nb, ix.B, V, nullptr, s);
+  // stage 2: part matrices, closed together
+  std::vector<int32_t *> mats(K);
+  std::vector<int32_t> ns(K);
+  std::vector<int64_t> lds(K);
+  for (int32_t p = 1; p <= K && rc == FAP_OK; p++) {
+    const int4 q = pinfo[p];
+    mats[p - 1] = ix.M + moff[p];
+    ns[p - 1] = q.y;
+    lds[p - 1] = ((int64_t)q.y + 3) & ~(int64_t)3;
+    rc = launch_gather(g, q.x, q.y, q.z, ix.B + (int64_t)q.w * V, V, mats[p - 1], lds[p - 1], s);
+  }
+  fap_fw_plan *plan = nullptr;
+  if (rc == FAP_OK) rc = fw_plan_create(ctx, K, mats.data(), ns.data(), lds.data(), 0, &plan);
+  if (rc == FAP_OK) rc = fw_plan_run(plan, s);
+  e = hipStreamSynchronize(s);
+  if (plan) fw_plan_destroy(plan);
+  (void)hipFree(d_src);
+  if (rc == FAP_OK && e != hipSuccess) rc = hip_fail(e, "query index build", __FILE__, __LINE__);
+  if (rc != FAP_OK) {
+    ix.release();
+    return rc;
+  }
+  ix.build_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+  ix.ready = true;
+  return FAP_OK;
+}
+
 void free_server(fap_server *S) {
   for (Lane &L : S->lanes) {
     if (L.ctx) (void)hipStreamSynchronize(L.ctx->stream);
@@ -538,12 +703,9 @@ void free_server(fap_server *S) {
     (void)hipStreamDestroy(S->client);
   }
   if (S->d_acc) (void)hipFree(S->d_acc);
-  if (S->resident) {
-    (void)hipFree(S->resident->row_offset);
-    (void)hipFree(S->resident->edges);
-    delete S->resident;
-  }
-  if (S->arena_base) (void)hipFree(S->arena_base);
+  S->index.release();
+  if (S->resident) (void)fap_graph_free(S->lanes.empty() ? nullptr : S->lanes[0].ctx, S->resident);
+ if (S->arena_base) (void)hipFree(S->arena_base);
   for (auto &kv : S->reqs) delete kv.second;
   delete S;
 }
@@ -563,6 +725,7 @@ void fap_serve_default_opts(fap_serve_opts *o) {
   o->max_batch = 64;
   o->max_sources = 256;
   o->poll_us = 20;
+  o->query_parts = 0;
 }
 
 int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out) {
@@ -571,7 +734,7 @@ int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out) {
   fap_serve_default_opts(&o);
   if (opts) o = *opts;
   if (o.hbm_budget < (1 << 20) || o.stage_bytes < (1 << 20) || o.lanes < 1 || o.lanes > 8 ||
-      o.max_batch < 1 || o.max_sources < 1 || o.poll_us < 1)
+      o.max_batch < 1 || o.max_sources < 1 || o.poll_us < 1 || o.query_parts < 0)
     return fail(FAP_E_ARG, "bad server options");
   o.stage_bytes = (int64_t)align_up((size_t)o.stage_bytes);
   fap_server *S = new fap_server();
@@ -632,22 +795,38 @@ int fap_serve_attach_graph(fap_server *S, int32_t V, int64_t E, const int32_t *r
       return fail(FAP_E_BUSY, "query requests in flight");
   fap_graph_dev *g = nullptr;
   FAP_TRY(fap_graph_upload(S->lanes[0].ctx, V, E, ro, col, w, &g));
-  // size every lane's SSSP state for a full wave now, so that no workspace
+  // size every lane's SSSP state for any wave now, so that no workspace
   // grows (and synchronizes the device) while waves run
-  const int64_t grid = std::min<int64_t>(S->opts.max_sources, S->lanes[0].ctx->num_cus * 4);
+  size_t ws_bytes = 0;
+  for (int32_t n = 1; n <= S->opts.max_sources; n++)
+    ws_bytes = std::max(ws_bytes, sssp_ws_bytes(S->lanes[0].ctx, g, n));
   for (Lane &L : S->lanes) {
-    const int rc = ensure_ws(L.ctx, L.ctx->ws_queue, (size_t)grid * 6 * (size_t)V * 4);
+    const int rc = ensure_ws(L.ctx, L.ctx->ws_queue, ws_bytes);
     if (rc != FAP_OK) {
       (void)fap_graph_free(S->lanes[0].ctx, g);
       return rc;
     }
   }
   if (S->resident) {
-    (void)hipFree(S->resident->row_offset);
-    (void)hipFree(S->resident->edges);
-    delete S->resident;
+    S->index.release();
+    (void)fap_graph_free(S->lanes[0].ctx, S->resident);
   }
   S->resident = g;
+  if (S->opts.query_parts > 0) {
+    HostCSR csr;
+    csr.V = V;
+    csr.E = E;
+    csr.row_offset.assign(ro, ro + V + 1);
+    csr.col.assign(col, col + E);
+    csr.w.assign(w, w + E);
+    const int32_t K = std::min<int32_t>(S->opts.query_parts, V);
+    const int rc = build_index(S, csr, K);
+    if (rc != FAP_OK) {
+      (void)fap_graph_free(S->lanes[0].ctx, S->resident);
+      S->resident = nullptr;
+      return rc;
+    }
+  }
   return FAP_OK;
 }
 
@@ -818,6 +997,8 @@ int fap_serve_get_stats(fap_server *S, fap_serve_stats *out) {
   out->running = running;
   out->arena_used = (int64_t)S->arena.used();
   out->arena_peak = (int64_t)S->arena.peak();
+  out->index_bytes = S->index.bytes;
+  out->index_build_ms = S->index.build_ms;
   if (!S->fatal.empty()) return fail(FAP_E_HIP, S->fatal);
   return FAP_OK;
 }

@@ -475,6 +475,30 @@ void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_
 
 }  // namespace
 
+// Launch shape of an SSSP over n_src sources: settings, per-workgroup state
+// (6 V words for single sources, 8 V + S V for groups of S, rows of 64
+// words) and grid (as many workgroups as source groups, up to 4 / 2 / 1 per
+// CU for 256 / 512 / 1024 threads, and an 8 GiB state budget).
+int sssp_layout(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
+                int *delta, int *group, int64_t *stride, int64_t *d_off) {
+  sssp_settings(ctx, g, n_src, threads, delta, group);
+  const int64_t vw = ((int64_t)g->V + 63) & ~(int64_t)63;
+  *d_off = 8 * vw;
+  *stride = *group == 1 ? 6 * vw : *d_off + (int64_t)*group * vw;
+  const int64_t units = (n_src + *group - 1) / *group;
+  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)8 << 30) / (*stride * 4));
+  const int per_cu = *threads == 256 ? 4 : *threads == 512 ? 2 : 1;
+  return (int)std::max<int64_t>(
+      1, std::min<int64_t>({units, (int64_t)ctx->num_cus * per_cu, by_mem}));
+}
+
+size_t sssp_ws_bytes(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src) {
+  int threads, delta, group;
+  int64_t stride, d_off;
+  const int grid = sssp_layout(ctx, g, n_src, &threads, &delta, &group, &stride, &d_off);
+  return (size_t)grid * stride * sizeof(uint32_t);
+}
+
 // Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
 // pointers) when d_rowptr is not null.  Sources are taken in groups of
 // `group` (1 = one source per workgroup pass, 4 or 8 = grouped kernel);
@@ -486,18 +510,8 @@ int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources
   const int V = g->V;
   if (V <= 0) return fail(FAP_E_ARG, "sssp: empty graph");
   int threads, delta, group;
-  sssp_settings(ctx, g, n_src, &threads, &delta, &group);
-  // per-workgroup state: 6 V words (single) or 8 V + S V words (grouped),
-  // rows of 64 words; as many workgroups as source groups, up to 4 / 2 / 1
-  // per CU (256 / 512 / 1024 threads) and an 8 GiB state budget
-  const int64_t vw = ((int64_t)V + 63) & ~(int64_t)63;
-  const int64_t d_off = 8 * vw;
-  const int64_t stride = group == 1 ? 6 * vw : d_off + (int64_t)group * vw;
-  const int64_t units = (n_src + group - 1) / group;
-  const int64_t by_mem = std::max<int64_t>(1, ((int64_t)8 << 30) / (stride * 4));
-  const int per_cu = threads == 256 ? 4 : threads == 512 ? 2 : 1;
-  const int grid = (int)std::max<int64_t>(
-      1, std::min<int64_t>({units, (int64_t)ctx->num_cus * per_cu, by_mem}));
+  int64_t stride, d_off;
+  const int grid = sssp_layout(ctx, g, n_src, &threads, &delta, &group, &stride, &d_off);
   FAP_TRY(ensure_ws(ctx, ctx->ws_queue, (size_t)grid * stride * sizeof(uint32_t)));
   uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
   const int32_t *src = d_sources;

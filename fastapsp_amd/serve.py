@@ -43,13 +43,14 @@ class Server:
     def __init__(self, device: int = 0, hbm_budget: int | None = None,
                  stage_bytes: int | None = None, lanes: int | None = None,
                  max_batch: int | None = None, max_sources: int | None = None,
-                 poll_us: int | None = None):
+                 poll_us: int | None = None, query_parts: int | None = None):
         lib = load()
         o = ServeOpts()
         lib.fap_serve_default_opts(C.byref(o))
         for k, v in (("hbm_budget", hbm_budget), ("stage_bytes", stage_bytes),
                      ("lanes", lanes), ("max_batch", max_batch),
-                     ("max_sources", max_sources), ("poll_us", poll_us)):
+                     ("max_sources", max_sources), ("poll_us", poll_us),
+                     ("query_parts", query_parts)):
             if v is not None:
                 setattr(o, k, int(v))
         self.opts = o

@@ -15,8 +15,9 @@
  * a wave from the queue -- requests of the head's kind in arrival order, as
  * many as fit in the arena, the staging and the wave limits -- and launches
  * it: for graphs, one dense build and one batched closure of every graph in
- * the wave; for queries, one SSSP launch over the distinct sources of every
- * request in the wave and one gather of the answers.  Waves on different
+ * the wave; for queries, one lookup launch over the query index (or, without
+ * one, one SSSP launch over the distinct sources of every request in the
+ * wave and one gather of the answers).  Waves on different
  * lanes run concurrently; a request admitted while others run overlaps
  * with them on the device.  A graph's n x n int32 result stays in the arena
  * until released; query answers are copied to the host on completion.
@@ -46,6 +47,9 @@ typedef struct fap_serve_opts {
   int32_t max_batch;   /* requests per wave */
   int32_t max_sources; /* distinct query sources per wave */
   int32_t poll_us;     /* completion polling period of the server thread */
+  int32_t query_parts; /* > 0: point queries use an index over this many parts of the
+                        * resident graph (built by fap_serve_attach_graph); 0: each
+                        * query batch runs SSSP from its sources */
 } fap_serve_opts;
 
 enum { FAP_REQ_GRAPH = 0, FAP_REQ_QUERY = 1 };
@@ -71,6 +75,8 @@ typedef struct fap_serve_stats {
   int64_t sssp_sources; /* distinct sources over completed query requests */
   int32_t max_wave;     /* largest wave so far */
   int32_t lanes;
+  int64_t index_bytes;  /* device memory of the query index (0 without one) */
+  double index_build_ms;
 } fap_serve_stats;
 
 /* defaults: 8 GiB arena, 64 MiB staging, 3 lanes, 64 requests and 256
@@ -81,7 +87,15 @@ int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out);
 int fap_serve_destroy(fap_server *s);
 
 /* the resident graph point queries run against (CSR, int32 weights in
- * [0, FAP_INF)); FAP_E_BUSY while query requests are queued or running */
+ * [0, FAP_INF)); FAP_E_BUSY while query requests are queued or running.
+ * With opts.query_parts = K > 0 it also builds the query index -- the
+ * reference's per-part pipeline without its last product: the graph is
+ * cut into K parts (preCondition), SSSP rows from every boundary vertex
+ * (sum of boundary counts x V int32) and every part's closed matrix stay
+ * resident, and a query (s, t) is one cell of s's part block: the part
+ * matrix entry when t is in s's part, the SSSP row when s is a boundary
+ * vertex, else the min over s's part boundary k of M[s][k] + row_k[t]
+ * (fap/fap.cpp:92-181).  Exact, like the full block. */
 int fap_serve_attach_graph(fap_server *s, int32_t V, int64_t E, const int32_t *row_offset,
                            const int32_t *col, const int32_t *w);
 

@@ -181,3 +181,50 @@ def test_serve_rejects_bad_requests():
         srv.release(rid)
         with pytest.raises(fa.FapError):
             srv.release(rid)                             # already released
+
+
+@pytest.mark.parametrize("R,K", [(40, 6), (33, 1), (24, 24)])
+def test_serve_indexed_queries_match_oracle(R, K):
+    """Point queries through the index (K parts of the resident graph): pairs
+    inside one part, from boundary and inner vertices, across parts, and to
+    unreachable vertices, all equal to the oracle's Dijkstra."""
+    ro, col, w = grid(R, 11)
+    V = len(ro) - 1
+    rng = np.random.default_rng(R + K)
+    with fa.Server(hbm_budget=64 << 20, lanes=2, query_parts=K) as srv:
+        srv.attach_graph(ro, col, w)
+        st = srv.stats()
+        assert st["index_bytes"] > 0 and st["index_build_ms"] > 0
+        batches = []
+        for k in (1, 64, 300, 7):
+            src = rng.integers(0, V, k).astype(np.int32)
+            dst = rng.integers(0, V, k).astype(np.int32)
+            dst[: k // 4] = src[: k // 4]                 # zero-length queries
+            batches.append((src, dst, srv.submit_queries(src, dst)))
+        allsrc = np.unique(np.concatenate([b[0] for b in batches]))
+        rows = dict(zip(allsrc.tolist(), O.batched_sssp(allsrc, ro, col, w)))
+        for src, dst, rid in batches:
+            srv.wait(rid)
+            want = np.array([rows[int(a)][b] for a, b in zip(src, dst)], np.int32)
+            np.testing.assert_array_equal(srv.answers(rid), want)
+            srv.release(rid)
+        assert srv.stats()["sssp_sources"] == 0
+
+
+def test_serve_indexed_all_pairs_small():
+    """Every (s, t) pair of a small directed graph through the index equals
+    the full APSP matrix."""
+    u, v, w = synth.rgg_graph(150, 5, 1, 50, 4)
+    ro, col, wt = synth.to_csr(150, u, v, w, True)
+    want = O.batched_sssp(np.arange(150, dtype=np.int32), ro, col, wt)
+    with fa.Server(hbm_budget=16 << 20, lanes=1, query_parts=5) as srv:
+        srv.attach_graph(ro, col, wt)
+        s, t = np.meshgrid(np.arange(150), np.arange(150), indexing="ij")
+        s, t = s.ravel().astype(np.int32), t.ravel().astype(np.int32)
+        ids = [srv.submit_queries(s[i:i + 2000], t[i:i + 2000]) for i in range(0, s.size, 2000)]
+        got = []
+        for rid in ids:
+            srv.wait(rid)
+            got.append(srv.answers(rid))
+            srv.release(rid)
+        np.testing.assert_array_equal(np.concatenate(got).reshape(150, 150), want)
