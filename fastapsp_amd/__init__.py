@@ -168,7 +168,7 @@ _SIGS = {
     "fap_tune_cache_get": (_int, [_vp, C.c_char_p, _i64, _pi32, _i32,
                                   C.POINTER(C.c_double)]),
     "fap_tune_key_fw": (_i64, [_i32]),
-    "fap_tune_key_sssp": (_i64, [_i32, _i64]),
+    "fap_tune_key_sssp": (_i64, [_i32, _i64, _i32]),
     "fap_tune_key_batch": (_i64, [_i32, _pi32]),
     "fap_ctx_set_tune_cache": (_int, [_vp, _vp]),
     "fap_tune_fw": (_int, [_vp, _i32, _i32]),
@@ -300,8 +300,8 @@ class TuneCache:
         return load().fap_tune_key_fw(int(n))
 
     @staticmethod
-    def key_sssp(V: int, E: int) -> int:
-        return load().fap_tune_key_sssp(int(V), int(E))
+    def key_sssp(V: int, E: int, sources: int) -> int:
+        return load().fap_tune_key_sssp(int(V), int(E), int(sources))
 
     @staticmethod
     def key_batch(sizes) -> int:

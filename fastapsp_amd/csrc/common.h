@@ -91,9 +91,9 @@ struct fap_ctx {
   // "auto" values defer to the tune cache, then to the defaults.
   int opt_fw_lookahead = -1;     // run round r+1's phases 1-2 beside round r's phase 3; -1 auto
   int opt_fw_tile = 0;           // FW block size 64 / 128; 0 auto
-  int opt_sssp_threads = 0;      // threads per SSSP workgroup: 256, 512, 1024; 0 auto
+  int opt_sssp_threads = 0;      // threads per SSSP workgroup: 256, 512, 1024; 0 auto (by count)
   int opt_sssp_delta = 0;        // near-far band; 0 auto (graph's own, cache, 8 x mean weight)
-  int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 auto
+  int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 = 1
   int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
   fap_tune_cache *tune = nullptr;  // not owned
 };

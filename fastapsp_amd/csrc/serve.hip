@@ -691,6 +691,13 @@ int build_index(fap_server *S, const HostCSR &csr, int32_t K) {
 }
 
 void free_server(fap_server *S) {
+  // the resident graph and its index first: freeing the graph uses lane 0's
+  // context, destroyed below
+  for (Lane &L : S->lanes)
+    if (L.ctx) (void)hipStreamSynchronize(L.ctx->stream);
+  S->index.release();
+  if (S->resident) (void)fap_graph_free(S->lanes.empty() ? nullptr : S->lanes[0].ctx, S->resident);
+  S->resident = nullptr;
   for (Lane &L : S->lanes) {
     if (L.ctx) (void)hipStreamSynchronize(L.ctx->stream);
     if (L.done) (void)hipEventDestroy(L.done);
@@ -703,9 +710,7 @@ void free_server(fap_server *S) {
     (void)hipStreamDestroy(S->client);
   }
   if (S->d_acc) (void)hipFree(S->d_acc);
-  S->index.release();
-  if (S->resident) (void)fap_graph_free(S->lanes.empty() ? nullptr : S->lanes[0].ctx, S->resident);
- if (S->arena_base) (void)hipFree(S->arena_base);
+  if (S->arena_base) (void)hipFree(S->arena_base);
   for (auto &kv : S->reqs) delete kv.second;
   delete S;
 }

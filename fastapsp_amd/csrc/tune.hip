@@ -207,7 +207,7 @@ int fap_tune_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *h_sources
   ctx->opt_sssp_threads = saved_t;
   ctx->opt_sssp_delta = saved_d;
   FAP_TRY(rc);
-  return fap_tune_cache_set(ctx->tune, "sssp", fap_tune_key_sssp(g->V, g->E), bp, 2, best);
+  return fap_tune_cache_set(ctx->tune, "sssp", fap_tune_key_sssp(g->V, g->E, n_src), bp, 2, best);
 }
 
 int fap_tune_batch(fap_ctx *ctx, int32_t count, const int32_t *n, int32_t reps) {

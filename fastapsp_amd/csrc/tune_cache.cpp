@@ -86,17 +86,18 @@ void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead) {
 void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
                    int *delta, int *group) {
   int32_t p[2] = {1024, 8};
-  const bool cached = tune_lookup(ctx, "sssp", fap_tune_key_sssp(g->V, g->E), p);
-  *threads = (ctx && ctx->opt_sssp_threads > 0) ? ctx->opt_sssp_threads : cached ? p[0] : 1024;
-  // Grouped sources cost ~1.2x the rounds of one source but share arc and
-  // distance loads: worth it once there are enough groups to fill the GPU.
-  if (ctx && ctx->opt_sssp_group > 0) {
-    *group = ctx->opt_sssp_group;
-  } else {
-    const int per_cu = *threads == 256 ? 4 : *threads == 512 ? 2 : 1;
-    const int64_t slots = (int64_t)(ctx ? ctx->num_cus : 256) * per_cu;
-    *group = (n_src + 7) / 8 >= slots ? 8 : (n_src + 3) / 4 >= slots ? 4 : 1;
-  }
+  const bool cached = tune_lookup(ctx, "sssp", fap_tune_key_sssp(g->V, g->E, n_src), p);
+  // Without a tuned entry: one 1024-thread workgroup per CU while the
+  // sources fit on the CUs, smaller workgroups (more per CU) beyond.
+  // Measured on the 512x512 grid: 74 sources 15.2 ms at 1024 threads vs
+  // 21.2 ms at 256; 3669 sources 187 ms at 256 threads vs 248 ms at 1024.
+  const int cus = ctx ? ctx->num_cus : 256;
+  const int by_count = n_src <= cus ? 1024 : n_src <= 2 * cus ? 512 : 256;
+  *threads = (ctx && ctx->opt_sssp_threads > 0) ? ctx->opt_sssp_threads : cached ? p[0] : by_count;
+  // Grouped sources only on request: on the same grid a group of 8 takes
+  // ~10x one source (its frontiers already keep a workgroup's lanes busy),
+  // so grouping gains nothing there.
+  *group = (ctx && ctx->opt_sssp_group > 0) ? ctx->opt_sssp_group : 1;
   if (ctx && ctx->opt_sssp_delta > 0) {
     *delta = ctx->opt_sssp_delta;
   } else if (g->delta > 0) {
@@ -229,11 +230,12 @@ int fap_tune_cache_load(fap_tune_cache *c, const char *path) {
 
 int64_t fap_tune_key_fw(int32_t n) { return ceil_log2((uint64_t)(n > 1 ? n : 1)); }
 
-int64_t fap_tune_key_sssp(int32_t V, int64_t E) {
+int64_t fap_tune_key_sssp(int32_t V, int64_t E, int32_t n_src) {
   const int64_t vb = ceil_log2((uint64_t)(V > 1 ? V : 1));
   const double deg = V > 0 ? (double)E / (double)V : 0.0;
   const int64_t db = (int64_t)lround(2.0 * log2(1.0 + deg));
-  return vb * 64 + (db < 63 ? db : 63);
+  const int64_t sb = ceil_log2((uint64_t)(n_src > 1 ? n_src : 1));
+  return ((vb * 64 + (db < 63 ? db : 63)) * 32) + (sb < 31 ? sb : 31);
 }
 
 int64_t fap_tune_key_batch(int32_t count, const int32_t *n) {

@@ -67,10 +67,10 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  * (fap_tune.h), then to the default in brackets:
  *   "fw_lookahead" -1/0/1   overlap FW round r+1's diagonal/panels with round r [0]
  *   "fw_tile"      0/64/128 FW block size [64]
- *   "sssp_threads" 0/256/512/1024  threads per SSSP workgroup (one source each) [1024]
+ *   "sssp_threads" 0/256/512/1024  threads per SSSP workgroup [1024 while the sources
+ *                           fit on the CUs, 512 up to twice that, 256 beyond]
  *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 x mean weight]
- *   "sssp_group"   0/1/4/8  sources per SSSP workgroup pass [8 or 4 when that still
- *                           gives a workgroup per CU slot, else 1]
+ *   "sssp_group"   0/1/4/8  sources per SSSP workgroup pass [1]
  *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);

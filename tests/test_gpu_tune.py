@@ -29,7 +29,7 @@ def test_tuners_record_entries(tmp_path):
         ro, col, wt = synth.to_csr(1600, u, v, w, False)
         dg = ctx.upload_graph(ro, col, wt)
         ctx.tune_sssp(dg, np.arange(0, 1600, 50), reps=1)
-        p, ms = cache.get("sssp", TuneCache.key_sssp(1600, len(col)))
+        p, ms = cache.get("sssp", TuneCache.key_sssp(1600, len(col), 32))
         assert p[0] in (256, 512, 1024) and p[1] in (2, 4, 8, 16, 32)
         sizes = [70, 130, 200, 64, 1]
         ctx.tune_batch(sizes, reps=1)
@@ -64,7 +64,7 @@ def test_sssp_follows_cache_and_stays_exact(params):
     src = np.array([0, 17, 450, 899], np.int32)
     with fa.Context(0) as ctx:
         cache = TuneCache()
-        cache.set("sssp", TuneCache.key_sssp(900, len(col)), list(params), 1.0)
+        cache.set("sssp", TuneCache.key_sssp(900, len(col), 4), list(params), 1.0)
         ctx.set_tune_cache(cache)
         dg = ctx.upload_graph(ro, col, wt)
         ds = ctx.to_device(src)

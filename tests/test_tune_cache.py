@@ -13,10 +13,12 @@ from fastapsp_amd import TuneCache
 def test_bucket_keys():
     kf = TuneCache.key_fw
     assert [kf(n) for n in (1, 2, 3, 64, 65, 8192, 8193)] == [0, 1, 2, 6, 7, 13, 14]
-    # V bucket * 64 + round(2 log2(1 + E/V))
-    assert TuneCache.key_sssp(262144, 952362) == 18 * 64 + round(2 * np.log2(1 + 952362 / 262144))
-    assert TuneCache.key_sssp(4096, 0) == 12 * 64
-    assert TuneCache.key_sssp(1, 0) == 0
+    # (V bucket * 64 + round(2 log2(1 + E/V))) * 32 + sources bucket
+    db = round(2 * np.log2(1 + 952362 / 262144))
+    assert TuneCache.key_sssp(262144, 952362, 74) == (18 * 64 + db) * 32 + 7
+    assert TuneCache.key_sssp(262144, 952362, 3669) == (18 * 64 + db) * 32 + 12
+    assert TuneCache.key_sssp(4096, 0, 1) == 12 * 64 * 32
+    assert TuneCache.key_sssp(1, 0, 1) == 0
     # 64-tile triples: one 65-vertex job has 2^3 = 8, plus a 64-vertex job's 1
     assert TuneCache.key_batch([65, 64]) == 4
     assert TuneCache.key_batch([]) == 0
