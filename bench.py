@@ -344,6 +344,11 @@ def bench_grid_batch(args):
     blks = [ctx.alloc((nmax, V)) for _ in range(min(G, len(parts)))]
     log(f"grid-batch: V={V} E={E} K={K} parts={len(parts)} group={G} "
         f"blocks={len(blks)}x{nmax}x{V}")
+    # SSSP launch shape for a group's boundary vertices, picked by the tuner
+    g0 = groups[0]
+    srcs = np.concatenate([st2ed[int(vo[p]): int(vo[p]) + int(bn[p])] for p in g0])
+    tuned = _tune(ctx, args, "sssp", fa.TuneCache.key_sssp(V, E, len(srcs)),
+                  lambda: ctx.tune_sssp(dg, srcs, reps=1))
 
     def run(batched):
         t = 0.0
@@ -389,6 +394,7 @@ def bench_grid_batch(args):
         "batched_over_plain": round(tp / tb, 2),
         "plain_GOp_s": round(ops / tp / 1e9, 1),
         "identical": bool(sp == sb),
+        "tuned": tuned,
     }
     print(json.dumps(out), flush=True)
     return 0 if sp == sb else 1
