@@ -141,6 +141,7 @@ _SIGS = {
     "fapg_get_path_from_one_point": (_int, [_vp, _i32, _pi32]),
     "fapg_path_p2p": (_int, [_vp, _i32, _i32, _pi32, _i32, _pi32]),
     "fap_pred_rows": (_int, [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _i64, _vp]),
+    "fap_subgraph_solve_groups": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, _vp, _i64, _i32]),
     "fap_subgraph_pred": (_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
     # serving (include/fap_serve.h)
@@ -505,6 +506,16 @@ class DeviceGraph:
         check(load().fap_subgraph_solve_batch(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
                                               _p32(B), R, ld_rb, stream),
               "fap_subgraph_solve_batch")
+
+    def subgraph_solve_groups(self, starts, n_subs, n_bdys, rowblocks, ld_rb, group_parts):
+        k = len(starts)
+        S = np.ascontiguousarray(starts, dtype=np.int32)
+        N = np.ascontiguousarray(n_subs, dtype=np.int32)
+        B = np.ascontiguousarray(n_bdys, dtype=np.int32)
+        R = (_vp * k)(*rowblocks)
+        check(load().fap_subgraph_solve_groups(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
+                                               _p32(B), R, ld_rb, int(group_parts)),
+              "fap_subgraph_solve_groups")
 
     def pred_rows(self, d_dist, ld, d_row_vertex, nrows, d_pred, ld_pred, stream=None):
         check(load().fap_pred_rows(self.ctx.handle, self.handle, d_dist, ld, d_row_vertex, nrows,

@@ -96,6 +96,15 @@ struct fap_ctx {
   int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 = 1
   int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
   fap_tune_cache *tune = nullptr;  // not owned
+  // pinned host staging for asynchronous descriptor uploads; free for reuse
+  // once ev_stage (recorded after the last upload from it) has completed
+  void *host_stage = nullptr;
+  size_t host_stage_bytes = 0;
+  hipEvent_t ev_stage = nullptr;
+  bool stage_pending = false;
+  // second context (own stream and workspaces) for two-stream pipelines of
+  // batched part solves; created on first use, owned
+  fap_ctx *peer = nullptr;
 };
 
 namespace fapi {
@@ -113,6 +122,11 @@ void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead);
 void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
                    int *delta, int *group);
 int batch_grid_per_cu(const fap_ctx *ctx, int64_t batch_key);
+// Pinned host staging of at least `bytes`, free to overwrite: waits for the
+// previous upload from it, grows it when needed (ctx.hip).
+int acquire_host_stage(fap_ctx *ctx, size_t bytes, void **out);
+// Marks the staging in use by an upload just enqueued on stream s.
+int release_host_stage(fap_ctx *ctx, hipStream_t s);
 // Device-side helpers implemented in ctx.hip.
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);

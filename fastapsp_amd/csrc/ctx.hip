@@ -48,6 +48,34 @@ int ensure_ws(fap_ctx *ctx, Workspace &ws, size_t bytes) {
   return FAP_OK;
 }
 
+int acquire_host_stage(fap_ctx *ctx, size_t bytes, void **out) {
+  if (ctx->stage_pending) {
+    FAP_HIP(hipEventSynchronize(ctx->ev_stage));
+    ctx->stage_pending = false;
+  }
+  if (ctx->host_stage_bytes < bytes) {
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    ctx->host_stage = nullptr;
+    ctx->host_stage_bytes = 0;
+    const size_t want = std::max<size_t>((bytes + 4095) & ~(size_t)4095, 1 << 16);
+    hipError_t e = hipHostMalloc(&ctx->host_stage, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->host_stage = nullptr;
+      return fail(FAP_E_NOMEM, "pinned staging allocation failed");
+    }
+    ctx->host_stage_bytes = want;
+  }
+  *out = ctx->host_stage;
+  return FAP_OK;
+}
+
+int release_host_stage(fap_ctx *ctx, hipStream_t s) {
+  FAP_HIP(hipEventRecord(ctx->ev_stage, s));
+  ctx->stage_pending = true;
+  return FAP_OK;
+}
+
 // Grid-stride fill; used instead of hipMemsetAsync so that the sequence can
 // be captured in a graph.
 __global__ void fill_i32_kernel(int32_t *__restrict__ d, int64_t n, int32_t v) {
@@ -156,6 +184,7 @@ int fap_ctx_create(int device, fap_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_la, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_panel, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "context setup", __FILE__, __LINE__);
@@ -167,6 +196,7 @@ int fap_ctx_create(int device, fap_ctx **out) {
 
 int fap_ctx_destroy(fap_ctx *ctx) {
   if (!ctx) return FAP_OK;
+  if (ctx->peer) (void)fap_ctx_destroy(ctx->peer);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   Workspace *all[] = {&ctx->ws_sub, &ctx->ws_queue, &ctx->ws_stamp, &ctx->ws_small,
@@ -174,6 +204,8 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   for (Workspace *w : all)
     if (w->ptr) (void)hipFree(w->ptr);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+  if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
   if (ctx->ev_poll) (void)hipEventDestroy(ctx->ev_poll);
   if (ctx->ev_fw_la) (void)hipEventDestroy(ctx->ev_fw_la);
   if (ctx->ev_fw_panel) (void)hipEventDestroy(ctx->ev_fw_panel);

@@ -24,6 +24,11 @@ namespace {
 constexpr float kRefInf = 1e8f;  // the reference's MAXVALUE (fap/utils/parameter.h:26)
 
 inline float to_ref_float(int32_t v) { return v >= FAP_INF ? kRefInf : (float)v; }
+
+// Parts per batched group in solve()/solveSubGraphs(): groups alternate
+// between two streams so one group's SSSP overlaps the previous group's
+// closure and products.
+constexpr int32_t kGroupParts = 32;
 }  // namespace
 
 struct fapGraph::Device {
@@ -332,8 +337,8 @@ int32_t fapGraph::run_batch(const std::vector<int32_t> &ids, const std::vector<i
     rb.push_back(blocks[i]);
   }
   if (starts.empty()) return 0;
-  if (fap_subgraph_solve_batch(dev_->ctx, dev_->g, (int32_t)starts.size(), starts.data(),
-                               ns.data(), bs.data(), rb.data(), num_vertexs, nullptr) != FAP_OK)
+  if (fap_subgraph_solve_groups(dev_->ctx, dev_->g, (int32_t)starts.size(), starts.data(),
+                                ns.data(), bs.data(), rb.data(), num_vertexs, kGroupParts) != FAP_OK)
     return fail(FAP_E_HIP, std::string("batched parts: ") + fap_last_error());
   return 0;
 }

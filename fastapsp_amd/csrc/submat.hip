@@ -6,8 +6,11 @@
 //            matrix with its own stride, no copy
 // scatter <- MysubMatDecode_path fap/utils/util_Myalgorithm.h:75-86
 // solve   <- fapGraph::run       fap/fap.cpp:92-181
+#include <string.h>
+
 #include <algorithm>
 
+#include "batch_desc.h"
 #include "common.h"
 
 namespace fapk {
@@ -159,62 +162,118 @@ int run_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int3
 // vertex of every part (rows written straight into each part's row block),
 // the part matrices gathered side by side and closed by one batched closure,
 // then the per-part min-plus products and scatters.  Synchronous.
-int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
-                             const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
-                             int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
+// Enqueues the batched solve of `count` parts on stream s without waiting:
+// one SSSP launch for every boundary vertex (a row pointer per source into
+// its part's row block), the part matrices gathered side by side in ws_sub,
+// one batched closure, then each part's product and scatter.  Sources, row
+// pointers and closure descriptors go up in one copy from the context's
+// pinned staging.  ctx's workspaces stay in use until s passes this work;
+// a later enqueue on the same stream is ordered after it.
+int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                           const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                           int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
   if (count <= 0) return FAP_OK;
   if ((int64_t)g->h_st2ed.size() != g->V)
     return fail(FAP_E_STATE, "graph has no partition (fap_graph_set_partition)");
-  // stage 1: sources and their output rows
-  std::vector<int32_t> src;
-  std::vector<int32_t *> rowptr;
-  for (int32_t p = 0; p < count; p++)
-    for (int32_t i = 0; i < n_bdys[p]; i++) {
-      src.push_back(g->h_st2ed[starts[p] + i]);
-      rowptr.push_back(rowblocks[p] + (int64_t)i * ld_rb);
-    }
-  const size_t nsrc = src.size();
-  const size_t src_bytes = (nsrc * sizeof(int32_t) + 255) & ~(size_t)255;
-  FAP_TRY(ensure_ws(ctx, ctx->ws_srcs, src_bytes + nsrc * sizeof(int32_t *) + 16));
-  int32_t *d_src = (int32_t *)ctx->ws_srcs.ptr;
-  int32_t **d_rowptr = (int32_t **)((char *)ctx->ws_srcs.ptr + src_bytes);
-  if (nsrc > 0) {
-    FAP_HIP(hipMemcpyAsync(d_src, src.data(), nsrc * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    FAP_HIP(hipMemcpyAsync(d_rowptr, rowptr.data(), nsrc * sizeof(int32_t *),
-                           hipMemcpyHostToDevice, s));
-    FAP_TRY(run_sssp_rows(ctx, g, d_src, (int32_t)nsrc, nullptr, 0, d_rowptr, s));
-  }
-  // stage 2: part matrices side by side, one batched closure
+  int64_t nsrc = 0, words = 0;
   std::vector<int64_t> ld_sub(count), off(count);
-  std::vector<int32_t *> subs(count);
-  int64_t words = 0;
   for (int32_t p = 0; p < count; p++) {
+    nsrc += n_bdys[p];
     ld_sub[p] = ((int64_t)n_subs[p] + 3) & ~(int64_t)3;
     off[p] = words;
     words += ld_sub[p] * n_subs[p];
   }
+  if (nsrc > INT32_MAX) return fail(FAP_E_ARG, "subgraph batch: too many sources");
   FAP_TRY(ensure_ws(ctx, ctx->ws_sub, (size_t)std::max<int64_t>(words, 1) * sizeof(int32_t)));
-  for (int32_t p = 0; p < count; p++) {
-    subs[p] = (int32_t *)ctx->ws_sub.ptr + off[p];
+  std::vector<int32_t *> subs(count);
+  for (int32_t p = 0; p < count; p++) subs[p] = (int32_t *)ctx->ws_sub.ptr + off[p];
+  FwBatchDesc fd;
+  build_fw_batch(count, subs.data(), n_subs, ld_sub.data(), &fd);
+  // staging layout: sources | row pointers | closure jobs | closure segments
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_rp = up(4 * (size_t)nsrc);
+  const size_t o_jobs = o_rp + up(sizeof(int32_t *) * (size_t)nsrc);
+  const size_t o_segs = o_jobs + up(sizeof(fapk::BatchJob) * fd.jobs.size());
+  const size_t total = o_segs + std::max<size_t>(sizeof(fapk::BatchSeg) * fd.segs.size(), 16);
+  FAP_TRY(ensure_ws(ctx, ctx->ws_srcs, total));
+  void *hs = nullptr;
+  FAP_TRY(acquire_host_stage(ctx, total, &hs));
+  char *h = (char *)hs;
+  int32_t *h_src = (int32_t *)h;
+  int32_t **h_rp = (int32_t **)(h + o_rp);
+  for (int32_t p = 0, k = 0; p < count; p++)
+    for (int32_t i = 0; i < n_bdys[p]; i++, k++) {
+      h_src[k] = g->h_st2ed[starts[p] + i];
+      h_rp[k] = rowblocks[p] + (int64_t)i * ld_rb;
+    }
+  memcpy(h + o_jobs, fd.jobs.data(), sizeof(fapk::BatchJob) * fd.jobs.size());
+  if (!fd.segs.empty()) memcpy(h + o_segs, fd.segs.data(), sizeof(fapk::BatchSeg) * fd.segs.size());
+  char *d = (char *)ctx->ws_srcs.ptr;
+  FAP_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
+  FAP_TRY(release_host_stage(ctx, s));
+  if (nsrc > 0)
+    FAP_TRY(run_sssp_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, nullptr, 0,
+                          (int32_t *const *)(d + o_rp), s));
+  for (int32_t p = 0; p < count; p++)
     FAP_TRY(launch_gather(g, starts[p], n_subs[p], n_bdys[p], rowblocks[p], ld_rb, subs[p],
                           ld_sub[p], s));
-  }
-  fap_fw_plan *plan = nullptr;
-  FAP_TRY(fw_plan_create(ctx, count, subs.data(), n_subs, ld_sub.data(), 0, &plan));
-  int rc = fw_plan_run(plan, s);
-  // stage 3 and the scatter, per part
-  for (int32_t p = 0; p < count && rc == FAP_OK; p++) {
+  FAP_TRY(launch_fw_batch(ctx, fd, (const fapk::BatchJob *)(d + o_jobs),
+                          (const fapk::BatchSeg *)(d + o_segs), s));
+  for (int32_t p = 0; p < count; p++) {
     const int32_t b = n_bdys[p];
-    rc = launch_minplus(subs[p] + (int64_t)b * ld_sub[p], ld_sub[p], rowblocks[p], ld_rb,
-                        rowblocks[p] + (int64_t)b * ld_rb, ld_rb, n_subs[p] - b, g->V, b, 0, s);
-    if (rc == FAP_OK)
-      rc = launch_scatter(g, starts[p], n_subs[p], n_subs[p], subs[p], ld_sub[p], rowblocks[p],
-                          ld_rb, s);
+    FAP_TRY(launch_minplus(subs[p] + (int64_t)b * ld_sub[p], ld_sub[p], rowblocks[p], ld_rb,
+                           rowblocks[p] + (int64_t)b * ld_rb, ld_rb, n_subs[p] - b, g->V, b, 0,
+                           s));
+    FAP_TRY(launch_scatter(g, starts[p], n_subs[p], n_subs[p], subs[p], ld_sub[p], rowblocks[p],
+                           ld_rb, s));
   }
-  hipError_t e = hipStreamSynchronize(s);
-  fw_plan_destroy(plan);
+  return FAP_OK;
+}
+
+int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                             const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                             int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
+  const int rc = enqueue_subgraph_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb, s);
+  const hipError_t e = hipStreamSynchronize(s);
   if (rc != FAP_OK) return rc;
   if (e != hipSuccess) return hip_fail(e, "subgraph batch", __FILE__, __LINE__);
+  return FAP_OK;
+}
+
+// Groups of `group_parts` parts, alternating between ctx's stream and its
+// peer context's: one group's SSSP (HBM-bound) runs beside the other's
+// closure and products (VALU-bound).  Synchronous.
+int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *starts,
+                        const int32_t *n_subs, const int32_t *n_bdys, int32_t *const *rowblocks,
+                        int64_t ld_rb, int32_t group_parts) {
+  if (count <= 0) return FAP_OK;
+  if (count <= group_parts)
+    return run_subgraph_solve_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb,
+                                    ctx->stream);
+  if (!ctx->peer) {
+    FAP_TRY(fap_ctx_create(ctx->device, &ctx->peer));
+  }
+  fap_ctx *peer = ctx->peer;
+  peer->opt_fw_lookahead = ctx->opt_fw_lookahead;
+  peer->opt_fw_tile = ctx->opt_fw_tile;
+  peer->opt_sssp_threads = ctx->opt_sssp_threads;
+  peer->opt_sssp_delta = ctx->opt_sssp_delta;
+  peer->opt_sssp_group = ctx->opt_sssp_group;
+  peer->opt_batch_grid = ctx->opt_batch_grid;
+  peer->tune = ctx->tune;
+  fap_ctx *c[2] = {ctx, peer};
+  int rc = FAP_OK;
+  for (int32_t first = 0, gi = 0; first < count && rc == FAP_OK; first += group_parts, gi++) {
+    fap_ctx *cc = c[gi & 1];
+    const int32_t n = std::min(group_parts, count - first);
+    rc = enqueue_subgraph_batch(cc, g, n, starts + first, n_subs + first, n_bdys + first,
+                                rowblocks + first, ld_rb, cc->stream);
+  }
+  const hipError_t e1 = hipStreamSynchronize(ctx->stream);
+  const hipError_t e2 = hipStreamSynchronize(peer->stream);
+  if (rc != FAP_OK) return rc;
+  if (e1 != hipSuccess) return hip_fail(e1, "subgraph groups", __FILE__, __LINE__);
+  if (e2 != hipSuccess) return hip_fail(e2, "subgraph groups", __FILE__, __LINE__);
   return FAP_OK;
 }
 
@@ -237,6 +296,21 @@ int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
   }
   return run_subgraph_solve_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb,
                                   pick_stream(ctx, stream));
+}
+
+int fap_subgraph_solve_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                              int32_t *const *rowblocks, int64_t ld_rb, int32_t group_parts) {
+  if (!ctx || !g) return fail(FAP_E_ARG, "null ctx/graph");
+  if (count < 0 || (count > 0 && (!starts || !n_subs || !n_bdys || !rowblocks)))
+    return fail(FAP_E_ARG, "subgraph groups: bad arguments");
+  if (group_parts < 1) return fail(FAP_E_ARG, "subgraph groups: group_parts < 1");
+  if (ld_rb < g->V) return fail(FAP_E_ARG, "subgraph groups: ld_rb < V");
+  for (int32_t p = 0; p < count; p++) {
+    FAP_TRY(check_part(g, starts[p], n_subs[p], n_bdys[p]));
+    if (n_subs[p] > 0 && !rowblocks[p]) return fail(FAP_E_ARG, "subgraph groups: null row block");
+  }
+  return run_subgraph_groups(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb, group_parts);
 }
 
 int fap_gather_submat(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int32_t n_sub,

@@ -193,6 +193,14 @@ int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                              int32_t *const *rowblocks, int64_t ld_rb, void *stream);
 
+/* fap_subgraph_solve_batch over groups of `group_parts` parts, alternating
+ * between the context's stream and a second context it creates on first
+ * use: one group's SSSP (HBM-bound) runs beside the previous group's
+ * closure and products (VALU-bound).  Same rows.  Synchronous. */
+int fap_subgraph_solve_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
+                              int32_t *const *rowblocks, int64_t ld_rb, int32_t group_parts);
+
 /* Predecessor ("path") rows for exact distance rows -- the path matrices the
  * reference builds when is_path_needed (fap/kernel/batch_sssp.h:82-83,110,
  * fap/kernel/floyd.h:44, fap/kernel/minplus.h:57-60,

@@ -340,3 +340,22 @@ def test_subgraph_solve_batch_rejects_bad_parts(gpu_ctx):
     dg.subgraph_solve_batch([], [], [], [], V)                     # empty batch is fine
     out.free()
     dg.close()
+
+
+@pytest.mark.parametrize("group_parts", [1, 3, 5, 100])
+def test_subgraph_solve_groups_matches_per_part(gpu_ctx, group_parts):
+    """Parts in groups on two streams: every row block equals the oracle's."""
+    R, K = 36, 11
+    V, ro, col, wt, gid, pc = partitioned_grid(R, K, 77)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    ids = list(range(1, K + 1))
+    outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), V)) for p in ids]
+    dg.subgraph_solve_groups([int(pc["ver_off"][p]) for p in ids],
+                             [int(pc["ver_num"][p]) for p in ids],
+                             [int(pc["bdy_num"][p]) for p in ids],
+                             [o.ptr for o in outs], V, group_parts)
+    for p, o in zip(ids, outs):
+        np.testing.assert_array_equal(o.get(), O.run_part(p, ro, col, wt, gid, pc))
+        o.free()
+    dg.close()
