@@ -39,6 +39,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.32 T int32 ops/s
+# HBM bytes per launch of the n = 8192 min-plus product, from two rocprofv3
+# counter runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a copy of
+# known size): 6.66e9 read + 0.27e9 written.
+MINPLUS_8192_TRAFFIC = 6.93e9
 HBM_PEAK_GBS = 8000.0
 
 
@@ -150,7 +154,8 @@ def bench_dense(args):
                    "n": n, "ops_per_step": ops_step},
         "roofline": {"bound": "valu", "kernel": "minplus_gemm_kernel",
                      "achieved": round(ach, 2), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(ach / VALU_PEAK_TOPS, 3), "traffic": None,
+                     "unit": "TOP/s", "frac": round(ach / VALU_PEAK_TOPS, 3),
+                     "traffic": MINPLUS_8192_TRAFFIC if n == 8192 else None,
                      "ms_per_launch": round(mp_avg, 3)},
         "roofline_fw": {"bound": "valu", "kernels": "fw_diag/fw_panel/fw_rest",
                         "achieved": round(ach_fw, 2), "peak": round(VALU_PEAK_TOPS, 2),

@@ -8,12 +8,15 @@
 //  * each matrix (job) is cut into 64 x 64 tiles and its blocked
 //    Floyd-Warshall becomes a sequence of phases -- diag(r), panels(r),
 //    rest(r) for r = 0 .. nb-1 -- each phase a set of independent tasks;
-//  * phase p of every job goes into launch p: a queue of tasks, largest
+//  * phase p of job j goes into launch p + off(j) (off = 0 by default;
+//    staggered offsets let some jobs' diagonal phases share a launch with
+//    others' remaining-tile phases, measured slower): a queue of tasks, largest
 //    jobs first, consumed by a persistent grid (workgroup w takes tasks w,
 //    w + grid, ...).  The queue is described by one segment per job phase
 //    (its first task and count), found by binary search, so descriptors
 //    grow with the number of phases, not tiles.  Tasks of one launch never
-//    depend on each other; the kernel boundary orders phase p after p-1;
+//    depend on each other; the kernel boundary orders a job's phase p
+//    after its phase p-1;
 //  * the launch sequence of a batch is a plan: descriptors uploaded once,
 //    launches captured in a hipGraph and replayed.
 //
@@ -125,7 +128,7 @@ namespace fapi {
 // decreasing size (longest processing time first), each job's phase one
 // segment of consecutive tasks.  A one-block job has only its diagonal.
 void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
-                    FwBatchDesc *out) {
+                    FwBatchDesc *out, int stagger) {
   out->jobs.resize(count);
   out->segs.clear();
   out->seg_off.assign(1, 0);
@@ -134,20 +137,30 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
   std::vector<int32_t> order(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
-  int max_phases = 0;
-  for (int j = 0; j < count; j++) {
+  // Job j's phase p goes to launch p + off[j].  Offsets cycle 0..stagger-1
+  // down the size order, so a launch holds the diagonal phase of some large
+  // jobs beside the (much larger) remaining-tile phase of others instead of
+  // a launch of diagonals alone.  Phases of one job stay in order.
+  std::vector<int32_t> off(count, 0);
+  stagger = std::max(1, stagger);
+  int launches = 0;
+  for (int i = 0; i < count; i++) {
+    const int j = order[i];
     const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
     out->jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], nb};
     out->ops += 2 * (int64_t)n[j] * n[j] * n[j];
-    max_phases = std::max(max_phases, nb > 1 ? 3 * nb : (n[j] > 0 ? 1 : 0));
+    const int phases = nb > 1 ? 3 * nb : (n[j] > 0 ? 1 : 0);
+    off[j] = i % stagger;
+    if (phases) launches = std::max(launches, off[j] + phases);
   }
-  for (int p = 0; p < max_phases; p++) {
+  for (int L = 0; L < launches; L++) {
     int32_t first = 0;
     for (int j : order) {
       if (n[j] <= 0) continue;
       const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
       const int phases = nb > 1 ? 3 * nb : 1;
-      if (p >= phases) continue;
+      const int p = L - off[j];
+      if (p < 0 || p >= phases) continue;
       const int r = nb > 1 ? p / 3 : 0;
       const int kind = nb > 1 ? p % 3 : fapk::TASK_DIAG;
       const int cnt = kind == fapk::TASK_DIAG    ? 1
