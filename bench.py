@@ -566,16 +566,17 @@ def bench_sssp_sweep(args):
         out = ctx.alloc((len(srcs), V))
         ref = None
         for threads in (256, 512, 1024):
-            for group in (1, 4, 8):
+            for group, binq in ((1, 0), (1, 3)):
                 ctx.set_option("sssp_threads", threads)
                 ctx.set_option("sssp_group", group)
+                ctx.set_option("sssp_bin", binq)
                 ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V,
                                                         ctx.stream), reps=1)
                 cs = ctx.checksum(out.ptr, len(srcs) * V)
                 ref = cs if ref is None else ref
                 byts = 4.0 * (V + 1) + 8.0 * len(col) + 4.0 * len(srcs) * V
                 r = {"graph": f"{R}x{R} w1..100", "parts": parts, "sources": len(srcs),
-                     "threads": threads, "group": group, "ms": round(ms, 3),
+                     "threads": threads, "group": group, "bin": binq, "ms": round(ms, 3),
                      "edges_per_s": len(srcs) * len(col) / (ms * 1e-3),
                      "algorithmic_GB_per_s": round(byts / (ms * 1e-3) / 1e9, 1),
                      "same_rows": cs == ref}
@@ -584,6 +585,7 @@ def bench_sssp_sweep(args):
     dg.close()
     ctx.set_option("sssp_threads", 0)
     ctx.set_option("sssp_group", 0)
+    ctx.set_option("sssp_bin", 3)
     return 0
 
 

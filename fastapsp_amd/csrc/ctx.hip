@@ -236,6 +236,9 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
     if (value != 0 && value != 1 && value != 4 && value != 8)
       return fail(FAP_E_ARG, "sssp_group must be 0 (auto), 1, 4 or 8");
     ctx->opt_sssp_group = (int)value;
+  } else if (k == "sssp_bin") {
+    if (value < 0 || value > 3) return fail(FAP_E_ARG, "sssp_bin must be 0..3");
+    ctx->opt_sssp_bin = (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
     ctx->opt_batch_grid = (int)value;
@@ -254,6 +257,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_delta") *value = ctx->opt_sssp_delta;
   else if (k == "batch_grid") *value = ctx->opt_batch_grid;
   else if (k == "sssp_group") *value = ctx->opt_sssp_group;
+  else if (k == "sssp_bin") *value = ctx->opt_sssp_bin;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

@@ -58,17 +58,74 @@ __device__ __forceinline__ void wave_push(bool push, uint32_t item, uint32_t *q,
   if (push) q[base + lane_prefix(mask)] = item;
 }
 
+// Reorders queue `q` (n entries) by vertex block (v >> bsh) into `tmp`:
+// counting sort over NBINS LDS bins, entries of one bin in arbitrary order.
+// Consecutive lanes then take vertices of the same block, so their distance
+// reads, row offsets and their neighbours' distances share cache lines.
+// Called by the whole workgroup after a barrier; ends with a barrier.
+constexpr int NBINS = 4096;
+
 template <int NF_THREADS>
+__device__ __forceinline__ void bin_queue(const uint32_t *__restrict__ q, uint32_t *__restrict__ tmp,
+                                          uint32_t n, int bsh, uint32_t *hist, int *wsum) {
+  constexpr int PER = NBINS / NF_THREADS;
+  constexpr int NW = NF_THREADS / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int b = tid; b < NBINS; b += NF_THREADS) hist[b] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += NF_THREADS) atomicAdd(&hist[q[i] >> bsh], 1u);
+  __syncthreads();
+  uint32_t loc[PER];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    loc[k] = hist[tid * PER + k];
+    sum += loc[k];
+  }
+  uint32_t x = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wave] = (int)x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t t = lane < NW ? (uint32_t)wsum[lane] : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)t, off);
+      if (lane >= off) t += y;
+    }
+    if (lane < NW) wsum[lane] = (int)t;
+  }
+  __syncthreads();
+  uint32_t base = (wave > 0 ? (uint32_t)wsum[wave - 1] : 0u) + x - sum;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    hist[tid * PER + k] = base;
+    base += loc[k];
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += NF_THREADS) {
+    const uint32_t v = q[i];
+    tmp[atomicAdd(&hist[v >> bsh], 1u)] = v;
+  }
+  __syncthreads();
+}
+
+template <int NF_THREADS, bool BIN>
 __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
     const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
     const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld,
-    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int delta) {
+    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int delta, int bsh,
+    uint32_t bin_min) {
   // Queue counters live in LDS in three sets of {next near, new far}.  Each
   // phase (a near round or a far split) appends into set ci, thread 0 zeroes
   // set ci+1 (mod 3) -- last read before the previous barrier, next written
   // after this one -- then one barrier, then every thread reads set ci.
   __shared__ uint32_t s_cnt[6];
   __shared__ int s_min;
+  __shared__ uint32_t s_hist[BIN ? NBINS : 1];
+  __shared__ int s_wsum[BIN ? NF_THREADS / 64 : 1];
   uint32_t *base = ws + (int64_t)blockIdx.x * ws_stride;
   NfState st;
   st.qa = base;
@@ -147,6 +204,12 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
         uint32_t *t = qa;
         qa = qb;
         qb = t;
+        if (BIN && n_near >= bin_min) {
+          bin_queue<NF_THREADS>(qa, qb, n_near, bsh, s_hist, s_wsum);
+          t = qa;
+          qa = qb;
+          qb = t;
+        }
       }
       if (n_far == 0) break;
       // ---- threshold = min far distance + delta ----
@@ -197,6 +260,12 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
       uint32_t *t = fa;
       fa = fb;
       fb = t;
+      if (BIN && n_near >= bin_min) {
+        bin_queue<NF_THREADS>(qa, qb, n_near, bsh, s_hist, s_wsum);
+        t = qa;
+        qa = qb;
+        qb = t;
+      }
     }
     // far piles are reused by the next source: tags make stale marks harmless
   }
@@ -468,9 +537,20 @@ void launch_group(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t
 template <int NT>
 void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t *src,
                    int n_src, int32_t *out, int64_t ld, int32_t *const *rowptr, uint32_t *ws,
-                   int64_t stride, int delta) {
-  hipLaunchKernelGGL(fapk::sssp_nf_kernel<NT>, dim3(grid), dim3(NT), 0, s, g->row_offset,
-                     g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride, delta);
+                   int64_t stride, int delta, int bin) {
+  // blocks of 2^bsh vertices, at least 64 (4 lines of distances), at most
+  // NBINS blocks; queues of at least bin_min entries are binned
+  int bsh = 6;
+  while (((int64_t)g->V >> bsh) >= fapk::NBINS) bsh++;
+  const uint32_t bin_min = bin >= 3 ? NT / 2 : bin == 2 ? NT : 2 * NT;
+  if (bin)
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       delta, bsh, bin_min);
+  else
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, false>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       delta, bsh, bin_min);
 }
 
 }  // namespace
@@ -516,9 +596,10 @@ int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources
   uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
   const int32_t *src = d_sources;
   if (group == 1) {
-    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
-    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
-    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta);
+    const int bin = ctx->opt_sssp_bin;
+    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
+    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
+    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
   } else if (group == 4) {
     if (threads == 256) launch_group<4, 256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
     else if (threads == 512) launch_group<4, 512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);

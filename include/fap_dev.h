@@ -71,6 +71,9 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *                           fit on the CUs, 512 up to twice that, 256 beyond]
  *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 x mean weight]
  *   "sssp_group"   0/1/4/8  sources per SSSP workgroup pass [1]
+ *   "sssp_bin"     0..3     reorder SSSP near queues by vertex block when they hold at
+ *                           least 2x / 1x / 0.5x the workgroup's threads (1 / 2 / 3);
+ *                           0 off [3]
  *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);

@@ -206,6 +206,31 @@ def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     dg.close()
 
 
+@pytest.mark.parametrize("threads,delta,binq", [(256, 0, 1), (256, 1000000, 3), (1024, 0, 2),
+                                                (512, 40, 3), (1024, 0, 0)])
+def test_sssp_binned_queues(gpu_ctx, threads, delta, binq):
+    """Queues reordered by vertex block (option sssp_bin, each threshold, and
+    off) on a 200x200 grid, where near queues exceed the thresholds."""
+    u, v, w = synth.grid_graph(200, 200, 0.05, 0.02, 1, 60, 12)
+    V = 40000
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    sources = np.array([0, 39999, 20100, 777, 777, 15000], np.int32)
+    dsrc = gpu_ctx.to_device(sources)
+    out = gpu_ctx.alloc((len(sources), V))
+    gpu_ctx.set_option("sssp_threads", threads)
+    gpu_ctx.set_option("sssp_delta", delta)
+    gpu_ctx.set_option("sssp_bin", binq)
+    try:
+        dg.sssp(dsrc.ptr, len(sources), out.ptr, V)
+    finally:
+        gpu_ctx.set_option("sssp_bin", 3)
+        gpu_ctx.set_option("sssp_threads", 0)
+        gpu_ctx.set_option("sssp_delta", 0)
+    np.testing.assert_array_equal(out.get(), O.batched_sssp(sources, ro, col, wt))
+    dg.close()
+
+
 @pytest.mark.parametrize("sssp_config", [(1024, 0, 8), (256, 3, 4)], indirect=True,
                          ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
 @pytest.mark.parametrize("name", ["grid8.npz", "rgg100.npz"])
