@@ -420,6 +420,147 @@ def bench_grid_batch(args):
     return 0 if same else 1
 
 
+def _serve_pool(count, V_res, seed):
+    """Request mix for the served workload: half whole-graph solves (random
+    geometric graphs, n log-uniform in [256, 4096], average degree 6, directed,
+    w 1..255), half batches of 1-64 (src, dst) queries on the resident graph."""
+    from fastapsp_amd import synth
+    rng = np.random.default_rng(seed)
+    kinds = rng.random(count) < 0.5
+    sizes = synth.loguniform_sizes(count, 256, 4096, seed)
+    pool = []
+    for i in range(count):
+        if kinds[i]:
+            n = int(sizes[i])
+            u, v, w = synth.rgg_graph(n, 6, 1, 255, 1000 + i)
+            pool.append(("graph", synth.to_csr(n, u, v, w, True)))
+        else:
+            k = int(rng.integers(1, 65))
+            pool.append(("queries", (rng.integers(0, V_res, k).astype(np.int32),
+                                     rng.integers(0, V_res, k).astype(np.int32))))
+    return pool
+
+
+def _serve_run(srv, pool, arrivals):
+    """Submits pool[i] at arrivals[i] seconds (None: all at once) from one
+    thread while this thread collects results in order; returns per-request
+    records (server-clock latencies) and the wall time."""
+    import threading
+    ids = [None] * len(pool)
+    ready = threading.Semaphore(0)
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, (kind, payload) in enumerate(pool):
+            if arrivals is not None:
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            ids[i] = srv.submit_graph(*payload) if kind == "graph" else srv.submit_queries(*payload)
+            ready.release()
+
+    th = threading.Thread(target=submitter)
+    t_start = time.perf_counter()
+    th.start()
+    recs = []
+    for i, (kind, payload) in enumerate(pool):
+        ready.acquire()
+        info = srv.wait(ids[i])
+        out = srv.checksum(ids[i]) if kind == "graph" else tuple(srv.answers(ids[i]).tolist())
+        srv.release(ids[i])
+        recs.append((kind, info, out))
+    th.join()
+    wall = time.perf_counter() - t_start
+    return recs, wall
+
+
+def _serve_summary(recs, wall, pool):
+    lat = np.array([r[1]["latency_us"] for r in recs]) / 1e3
+    lg = np.array([r[1]["latency_us"] for r in recs if r[0] == "graph"]) / 1e3
+    lq = np.array([r[1]["latency_us"] for r in recs if r[0] == "queries"]) / 1e3
+    ops = sum(2.0 * (len(p[1][0]) - 1) ** 3 for p in pool if p[0] == "graph")
+    first = min(r[1]["submit_us"] for r in recs)
+    last = max(r[1]["done_us"] for r in recs)
+    span = (last - first) / 1e6
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 3) if a.size else None
+
+    return {"requests": len(recs), "span_s": round(span, 3), "req_per_s": round(len(recs) / span, 1),
+            "GOp_s": round(ops / span / 1e9, 1),
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "graph_p50_ms": pct(lg, 50), "graph_p99_ms": pct(lg, 99),
+            "query_p50_ms": pct(lq, 50), "query_p99_ms": pct(lq, 99),
+            "mean_wave": round(float(np.mean([r[1]["wave_size"] for r in recs])), 2)}
+
+
+def bench_serve(args):
+    """Served solves: Poisson arrivals of whole-graph APSP requests and point
+    query batches against a resident 512x512 road-like grid, admitted by an
+    HBM budget.  First a burst measures the saturation rate; then the same
+    request mix arrives at a fixed rate (--rate-frac of saturation) on the
+    continuous-batching server (queries through the index over
+    --query-parts parts) and on a plain one (one lane, one request per
+    launch sequence, an SSSP per query batch).  Results of both runs must
+    be identical."""
+    import fastapsp_amd as fa
+    from fastapsp_amd import synth
+    R = args.grid
+    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
+    ro, col, wt = synth.to_csr(R * R, u, v, w, False)
+    pool = _serve_pool(args.requests, R * R, 5)
+    log(f"serve: {len(pool)} requests ({sum(p[0] == 'graph' for p in pool)} graphs), "
+        f"resident V={R * R}")
+    budget = int(args.hbm_gib * (1 << 30))
+    out = {}
+    with fa.Server(hbm_budget=budget, lanes=args.lanes, query_parts=args.query_parts) as srv:
+        t0 = time.perf_counter()
+        srv.attach_graph(ro, col, wt)
+        attach_s = time.perf_counter() - t0
+        ist = srv.stats()
+        log(f"  attach {attach_s:.2f} s (index {ist['index_bytes'] / 2**30:.1f} GiB, "
+            f"{ist['index_build_ms']:.0f} ms)")
+        _serve_run(srv, pool[:16], None)  # warm-up
+        recs, wall = _serve_run(srv, pool, None)
+        sat = _serve_summary(recs, wall, pool)
+        ref = [r[2] for r in recs]
+        lam = args.rate_frac * sat["req_per_s"]
+        log(f"  saturation {sat['req_per_s']} req/s; arrivals at {lam:.1f} req/s")
+        gaps = np.random.default_rng(11).exponential(1.0 / lam, len(pool))
+        arrivals = np.cumsum(gaps) - gaps[0]
+        recs, wall = _serve_run(srv, pool, arrivals)
+        served = _serve_summary(recs, wall, pool)
+        same = [r[2] for r in recs] == ref
+        st = srv.stats()
+    log(f"  served: {served}")
+    with fa.Server(hbm_budget=budget, lanes=1, max_batch=1) as srv:
+        srv.attach_graph(ro, col, wt)
+        _serve_run(srv, pool[:16], None)
+        recs, wall = _serve_run(srv, pool, arrivals)
+        plain = _serve_summary(recs, wall, pool)
+        same = same and [r[2] for r in recs] == ref
+    log(f"  plain: {plain}")
+    out = {
+        "metric": "served p50/p99 solve latency at a fixed arrival rate", "value": served["p99_ms"],
+        "unit": "ms (p99)", "n_gpus": 1, "steps": len(pool), "warmup": 16,
+        "ms_per_step": round(1e3 * served["span_s"] / len(pool), 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"Poisson stream, half RGG APSP (n logU[256,4096], deg 6, "
+                               f"w 1..255), half 1-64 point queries on a {R}x{R} grid "
+                               f"(V={R * R}); HBM budget {args.hbm_gib} GiB, "
+                               f"{args.lanes} lanes",
+                   "arrival_rate_req_s": round(lam, 2), "rate_frac_of_saturation": args.rate_frac},
+        "saturation": sat, "served": served, "plain_same_rate": plain,
+        "served_over_plain_p99": round(plain["p99_ms"] / served["p99_ms"], 2),
+        "identical": bool(same),
+        "server_stats": {k: st[k] for k in ("waves", "max_wave", "arena_peak", "index_bytes",
+                                            "index_build_ms")},
+        "query_index": {"parts": args.query_parts, "attach_s": round(attach_s, 3)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
 def bench_c1(args):
     """Plain method, small: 64x64 grid (5% of edges removed), unit weights,
     K=8, the full 4096 x 4096 matrix through the drop-in fapGraph::solve(),

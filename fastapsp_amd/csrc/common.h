@@ -95,7 +95,7 @@ struct fap_ctx {
   int opt_sssp_delta = 0;        // near-far band; 0 auto (graph's own, cache, 8 x mean weight)
   int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 = 1
   int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
-  int opt_sssp_bin = 3;          // reorder SSSP queues by vertex block from 2NT/NT/NT2 entries (1/2/3); 0 off
+  int opt_sssp_bin = 3;          // reorder SSSP queues by vertex block from 2NT / NT / NT/2 entries (1/2/3); 0 off
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse
   // once ev_stage (recorded after the last upload from it) has completed

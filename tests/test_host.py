@@ -163,3 +163,20 @@ def test_bad_inputs_are_errors(tmp_path, monkeypatch):
     with pytest.raises(fa.FapError):
         g.solve()                                      # before preCondition
     g.close()
+
+
+def test_bench_workloads_are_defined():
+    """Every --workload choice of bench.py dispatches to a defined function,
+    and --gpus above 1 is refused with a message (no device touched)."""
+    import importlib.util
+    import subprocess
+    import sys
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for name in ("dense", "copy", "c1", "grid_pipeline", "grid_batch", "fw_sweep", "sssp_sweep",
+                 "batch", "serve"):
+        assert callable(getattr(bench, "bench_" + name)), name
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
