@@ -683,8 +683,8 @@ def bench_fw_sweep(args):
 
 
 def bench_sssp_sweep(args):
-    """Multi-source SSSP time per (threads, sources per workgroup pass) on the
-    512x512 grid (w 1..100, K=256 partition): the boundary vertices of one
+    """Multi-source SSSP time per (threads, workgroups per CU, state layout,
+    queue binning) on the 512x512 grid (w 1..100, K=256 partition): the boundary vertices of one
     part (plain per-part stage 1) and of 32 parts (a batched group)."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
@@ -706,10 +706,12 @@ def bench_sssp_sweep(args):
         dsrc = ctx.to_device(srcs)
         out = ctx.alloc((len(srcs), V))
         ref = None
-        for threads in (256, 512, 1024):
-            for group, binq in ((1, 0), (1, 3)):
+        for threads, wgs in ((256, 2), (256, 4), (256, 8), (512, 1), (512, 2), (512, 4),
+                             (1024, 1), (1024, 2)):
+            for pack, binq in ((1, 3), (1, 0), (2, 3)):
                 ctx.set_option("sssp_threads", threads)
-                ctx.set_option("sssp_group", group)
+                ctx.set_option("sssp_wgs", wgs)
+                ctx.set_option("sssp_pack", pack)
                 ctx.set_option("sssp_bin", binq)
                 ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V,
                                                         ctx.stream), reps=1)
@@ -717,7 +719,8 @@ def bench_sssp_sweep(args):
                 ref = cs if ref is None else ref
                 byts = 4.0 * (V + 1) + 8.0 * len(col) + 4.0 * len(srcs) * V
                 r = {"graph": f"{R}x{R} w1..100", "parts": parts, "sources": len(srcs),
-                     "threads": threads, "group": group, "bin": binq, "ms": round(ms, 3),
+                     "threads": threads, "wgs_per_cu": wgs, "pack": pack, "bin": binq,
+                     "ms": round(ms, 3),
                      "edges_per_s": len(srcs) * len(col) / (ms * 1e-3),
                      "algorithmic_GB_per_s": round(byts / (ms * 1e-3) / 1e9, 1),
                      "same_rows": cs == ref}
@@ -725,7 +728,8 @@ def bench_sssp_sweep(args):
         out.free()
     dg.close()
     ctx.set_option("sssp_threads", 0)
-    ctx.set_option("sssp_group", 0)
+    ctx.set_option("sssp_wgs", 0)
+    ctx.set_option("sssp_pack", 0)
     ctx.set_option("sssp_bin", 3)
     return 0
 

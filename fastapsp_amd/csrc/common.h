@@ -96,6 +96,8 @@ struct fap_ctx {
   int opt_sssp_group = 0;        // sources per SSSP workgroup pass: 1, 4, 8; 0 = 1
   int opt_batch_grid = 0;        // batched-closure workgroups per CU per launch; 0 auto
   int opt_sssp_bin = 3;          // reorder SSSP queues by vertex block from 2NT / NT / NT/2 entries (1/2/3); 0 off
+  int opt_sssp_pack = 1;         // SSSP state: 64-bit words (1), in-row 32-bit words (2)
+  int opt_sssp_wgs = 0;          // SSSP workgroups per CU: 1..8; 0 = 1024 threads per CU
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse
   // once ev_stage (recorded after the last upload from it) has completed

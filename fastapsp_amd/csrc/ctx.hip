@@ -239,6 +239,12 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "sssp_bin") {
     if (value < 0 || value > 3) return fail(FAP_E_ARG, "sssp_bin must be 0..3");
     ctx->opt_sssp_bin = (int)value;
+  } else if (k == "sssp_pack") {
+    if (value < 0 || value > 2) return fail(FAP_E_ARG, "sssp_pack must be 0 (auto), 1 or 2");
+    ctx->opt_sssp_pack = value == 0 ? 1 : (int)value;
+  } else if (k == "sssp_wgs") {
+    if (value < 0 || value > 8) return fail(FAP_E_ARG, "sssp_wgs must be 0 (auto) .. 8");
+    ctx->opt_sssp_wgs = (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
     ctx->opt_batch_grid = (int)value;
@@ -258,6 +264,8 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "batch_grid") *value = ctx->opt_batch_grid;
   else if (k == "sssp_group") *value = ctx->opt_sssp_group;
   else if (k == "sssp_bin") *value = ctx->opt_sssp_bin;
+  else if (k == "sssp_pack") *value = ctx->opt_sssp_pack;
+  else if (k == "sssp_wgs") *value = ctx->opt_sssp_wgs;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

@@ -11,18 +11,19 @@
 // Method (near-far, Davidson et al. 2014, restated): each workgroup owns one
 // source at a time and keeps, in global memory, two "near" queues (current
 // and next round) and two "far" piles.  A round relaxes the out-arcs of every
-// near vertex u (strict: atomicMin(dist[v], dist[u] + w) only when a plain
-// read shows the candidate is smaller).  An improved v goes to the next near
-// queue if its new distance is below the threshold, otherwise to the far pile.
-// When the near queue drains, the threshold moves to min(far distance) + delta
-// and the far pile is split.  Vertices are therefore settled in near
-// distance order, as Dijkstra's buckets would, and re-relaxations are limited
-// to a band of width delta.
+// near vertex u (strictly: dist[v] is lowered to dist[u] + w only when that is
+// smaller).  An improved v goes to the next near queue if its new distance is
+// below the threshold, otherwise to the far pile.  When the near queue
+// drains, the threshold moves to min(far distance) + delta and the far pile
+// is split.  Vertices are therefore settled in near distance order, as
+// Dijkstra's buckets would, and re-relaxations are limited to a band of width
+// delta.
 //
 // Queue appends are compacted per wavefront: a ballot of the pushing lanes,
 // one LDS atomicAdd per wave, lane slot = mbcnt of the ballot.  Duplicates
-// are suppressed by a per-vertex round stamp (near) and a per-source tag
-// (far); both arrays are never cleared, the stamps only grow.
+// are suppressed by queue state kept beside each distance, in the same word,
+// so an improvement is one compare-and-swap on one cache line (see the two
+// kernels below for the two word layouts).
 //
 // Every loop is bounded: a vertex enters a queue only after a strict
 // improvement of its distance, and each far split raises the threshold above
@@ -34,12 +35,6 @@
 
 namespace fapk {
 
-
-struct NfState {
-  uint32_t *qa, *qb, *fa, *fb;  // near current/next, far current/next (V each)
-  int32_t *stamp;               // near round stamp (V)
-  int32_t *farmark;             // far-pile tag (V)
-};
 
 __device__ __forceinline__ unsigned lane_prefix(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
@@ -112,51 +107,65 @@ __device__ __forceinline__ void bin_queue(const uint32_t *__restrict__ q, uint32
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Packed state (the default): a vertex's distance and queue tag in one
+// 64-bit word (distance high, tag low), changed together by one
+// compare-and-swap.  A relaxation touches one line of per-source state, not
+// three (distance, round stamp and far tag in separate arrays measured 1.4x
+// slower), and an improvement costs one atomic instead of two.  The working
+// row belongs to the workgroup; the finished distances are copied to the
+// output row with coalesced stores.
+//
+// Tags come from a per-source counter (every source rewrites all words):
+//   0       never reached
+//   R       pushed onto the next near queue by near round R, or moved to the
+//           near queue by the far split that handed out R
+//   ft      live in the current far pile; each split gives the entries that
+//           stay far a new ft, and entries whose word carries another tag
+//           (near-improved since they were pushed) are dropped
+// An improvement to nd pushes v to the next near queue when nd < threshold
+// and the old tag is not this round's R, to the far pile when nd >= threshold
+// and the old tag is not ft.  Distances only decrease and the threshold is
+// fixed between splits, so a vertex is live in the far pile at most once and
+// the split rewrites words with plain stores.  Each failed compare-and-swap
+// means another lane lowered the distance, so the retry loop is bounded.
+__device__ __forceinline__ unsigned long long nf_word(int d, uint32_t tag) {
+  return ((unsigned long long)(uint32_t)d << 32) | tag;
+}
+
+__device__ __forceinline__ int nf_dist(unsigned long long w) { return (int)(w >> 32); }
+
 template <int NF_THREADS, bool BIN>
 __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
     const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
     const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld,
-    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int delta, int bsh,
-    uint32_t bin_min) {
-  // Queue counters live in LDS in three sets of {next near, new far}.  Each
-  // phase (a near round or a far split) appends into set ci, thread 0 zeroes
-  // set ci+1 (mod 3) -- last read before the previous barrier, next written
-  // after this one -- then one barrier, then every thread reads set ci.
+    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int64_t q_off,
+    int delta, int bsh, uint32_t bin_min) {
   __shared__ uint32_t s_cnt[6];
   __shared__ int s_min;
   __shared__ uint32_t s_hist[BIN ? NBINS : 1];
   __shared__ int s_wsum[BIN ? NF_THREADS / 64 : 1];
   uint32_t *base = ws + (int64_t)blockIdx.x * ws_stride;
-  NfState st;
-  st.qa = base;
-  st.qb = base + V;
-  st.fa = base + 2 * (int64_t)V;
-  st.fb = base + 3 * (int64_t)V;
-  st.stamp = (int32_t *)(base + 4 * (int64_t)V);
-  st.farmark = (int32_t *)(base + 5 * (int64_t)V);
+  unsigned long long *W = (unsigned long long *)base;  // V words, 8-byte aligned
+  uint32_t *const q0 = base + q_off;
   const int tid = threadIdx.x;
-  // stamps are per workgroup and only grow: start from 0 once per launch
-  for (int v = tid; v < V; v += NF_THREADS) {
-    st.stamp[v] = 0;
-    st.farmark[v] = -1;
-  }
   if (tid < 6) s_cnt[tid] = 0;
-  int round_tag = 0;
   int ci = 0;
   for (int si = blockIdx.x; si < S; si += gridDim.x) {
     int32_t *dist = rowptr ? rowptr[si] : out + (int64_t)si * ld;
     const int src = sources[si];
-    for (int v = tid; v < V; v += NF_THREADS) dist[v] = (v == src) ? 0 : 0x3FFFFFFF;
-    uint32_t *qa = st.qa, *qb = st.qb, *fa = st.fa, *fb = st.fb;
+    for (int v = tid; v < V; v += NF_THREADS)
+      W[v] = v == src ? nf_word(0, 1u) : nf_word(FAP_INF, 0u);
+    uint32_t *qa = q0, *qb = q0 + V, *fa = q0 + 2 * (int64_t)V, *fb = q0 + 3 * (int64_t)V;
     uint32_t n_near = 1, n_far = 0;
     if (tid == 0) qa[0] = (uint32_t)src;
+    uint32_t tags = 2;  // last tag handed out: 1 = the source's, 2 = the first far pile's
+    uint32_t ft = 2;
     int thr = delta;
-    const int tag = si;  // distinct per source handled by this workgroup
     __syncthreads();
     for (;;) {
-      // ---- drain the near queue, one barrier per round ----
       while (n_near > 0) {
-        round_tag++;
+        const uint32_t R = ++tags;
         uint32_t *cn = &s_cnt[2 * ci];
         for (uint32_t i0 = 0; i0 < n_near; i0 += NF_THREADS) {
           const uint32_t i = i0 + tid;
@@ -164,7 +173,7 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
           int du = 0, e = 0, e_end = 0;
           if (valid) {
             const int u = (int)qa[i];
-            du = dist[u];
+            du = nf_dist(W[u]);
             e = row_offset[u];
             e_end = row_offset[u + 1];
           }
@@ -175,17 +184,20 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
               const int2 ed = edges[e];
               e++;
               const int nd = du + ed.y;
-              int32_t *dp = dist + ed.x;
-              if (nd < *dp) {
-                const int old = atomicMin(dp, nd);
-                if (nd < old) {
+              unsigned long long *wp = W + ed.x;
+              unsigned long long cur = *wp;
+              const bool near = nd < thr;
+              const unsigned long long nw = nf_word(nd, near ? R : ft);
+              while (nd < nf_dist(cur)) {
+                const unsigned long long prev = atomicCAS(wp, cur, nw);
+                if (prev == cur) {
+                  const uint32_t old_tag = (uint32_t)cur;
                   item = (uint32_t)ed.x;
-                  if (nd < thr) {
-                    pn = atomicMax(st.stamp + ed.x, round_tag) < round_tag;
-                  } else {
-                    pf = atomicExch(st.farmark + ed.x, tag) != tag;
-                  }
+                  if (near) pn = old_tag != R;
+                  else pf = old_tag != ft;
+                  break;
                 }
+                cur = prev;
               }
             }
             wave_push(pn, item, qb, &cn[0]);
@@ -212,13 +224,13 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
         }
       }
       if (n_far == 0) break;
-      // ---- threshold = min far distance + delta ----
+      // ---- threshold = min live far distance + delta ----
       if (tid == 0) s_min = 0x7FFFFFFF;
       __syncthreads();
       int lmin = 0x7FFFFFFF;
       for (uint32_t i = tid; i < n_far; i += NF_THREADS) {
-        const int v = (int)fa[i];
-        if (st.farmark[v] == tag) lmin = min(lmin, dist[v]);
+        const unsigned long long w = W[fa[i]];
+        if ((uint32_t)w == ft) lmin = min(lmin, nf_dist(w));
       }
       for (int off = 32; off > 0; off >>= 1) lmin = min(lmin, __shfl_down(lmin, off));
       if ((tid & 63) == 0) atomicMin(&s_min, lmin);
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
       const int fmin = s_min;
       if (fmin == 0x7FFFFFFF) break;  // every far entry was stale
       thr = fmin + delta;
-      round_tag++;
+      const uint32_t rs = ++tags, fn = ++tags;
       // ---- split the far pile: below thr -> near, else stays far ----
       uint32_t *cn = &s_cnt[2 * ci];
       for (uint32_t i0 = 0; i0 < n_far; i0 += NF_THREADS) {
@@ -236,13 +248,180 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
         if (i < n_far) {
           const int v = (int)fa[i];
           item = (uint32_t)v;
-          if (st.farmark[v] == tag) {
-            if (dist[v] < thr) {
-              st.farmark[v] = -1;
-              pn = atomicMax(st.stamp + v, round_tag) < round_tag;
-            } else {
-              pf = true;
+          const unsigned long long w = W[v];
+          if ((uint32_t)w == ft) {
+            const int d = nf_dist(w);
+            pn = d < thr;
+            pf = !pn;
+            W[v] = nf_word(d, pn ? rs : fn);
+          }
+        }
+        wave_push(pn, item, qa, &cn[0]);
+        wave_push(pf, item, fb, &cn[1]);
+      }
+      ft = fn;
+      const int cz = ci == 2 ? 0 : ci + 1;
+      if (tid == 0) {
+        s_cnt[2 * cz] = 0;
+        s_cnt[2 * cz + 1] = 0;
+      }
+      __syncthreads();
+      n_near = cn[0];
+      n_far = cn[1];
+      ci = cz;
+      uint32_t *t = fa;
+      fa = fb;
+      fb = t;
+      if (BIN && n_near >= bin_min) {
+        bin_queue<NF_THREADS>(qa, qb, n_near, bsh, s_hist, s_wsum);
+        t = qa;
+        qa = qb;
+        qb = t;
+      }
+    }
+    // every exit follows a barrier, and each thread copies (and the next
+    // source re-initialises) the same words it owns: v = tid mod threads
+    for (int v = tid; v < V; v += NF_THREADS) dist[v] = nf_dist(W[v]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// In-row state: distances stay below INF = 2^30 - 1, so a vertex's distance
+// and two state bits fit one 32-bit word, d << 2 | state, kept in the output
+// row itself (converted to plain distances in place at the end).  Per-source
+// state is 4 bytes per vertex, and the workspace holds only the queues.
+//   NQ  in the next near queue: set by a near push, cleared when the vertex
+//       is popped -- by the atomic and-not that also reads its distance, so
+//       an improvement either lands before the pop (and is used by it) or
+//       sees NQ clear and queues the vertex again
+//   NF  live in the far pile: set by a far push, cleared by a near
+//       improvement (the pile entry turns stale) or by the split that moves
+//       the vertex to the near queue
+// A vertex is in a queue at most once (NQ) and live in the far pile at most
+// once (NF), so every queue and pile holds at most V entries.  When a near
+// round ends every queued vertex has been popped, so NQ is clear everywhere
+// and the split may set it with plain stores.
+constexpr uint32_t ST_NQ = 1u, ST_NF = 2u;
+
+// Rows reached through a row-pointer array are global memory; saying so lets
+// the state atomics and loads use global (not flat) instructions.
+typedef __attribute__((address_space(1))) uint32_t gu32;
+
+template <int NF_THREADS, bool BIN>
+__global__ __launch_bounds__(NF_THREADS) void sssp_nfs_kernel(
+    const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
+    const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld,
+    int32_t *const *__restrict__ rowptr, uint32_t *ws, int64_t ws_stride, int delta, int bsh,
+    uint32_t bin_min) {
+  __shared__ uint32_t s_cnt[6];
+  __shared__ int s_min;
+  __shared__ uint32_t s_hist[BIN ? NBINS : 1];
+  __shared__ int s_wsum[BIN ? NF_THREADS / 64 : 1];
+  uint32_t *const q0 = ws + (int64_t)blockIdx.x * ws_stride;
+  const int tid = threadIdx.x;
+  if (tid < 6) s_cnt[tid] = 0;
+  int ci = 0;
+  for (int si = blockIdx.x; si < S; si += gridDim.x) {
+    gu32 *W = (gu32 *)(rowptr ? rowptr[si] : out + (int64_t)si * ld);
+    const int src = sources[si];
+    for (int v = tid; v < V; v += NF_THREADS)
+      W[v] = v == src ? ST_NQ : (uint32_t)FAP_INF << 2;
+    uint32_t *qa = q0, *qb = q0 + V, *fa = q0 + 2 * (int64_t)V, *fb = q0 + 3 * (int64_t)V;
+    uint32_t n_near = 1, n_far = 0;
+    if (tid == 0) qa[0] = (uint32_t)src;
+    int thr = delta;
+    __syncthreads();
+    for (;;) {
+      while (n_near > 0) {
+        uint32_t *cn = &s_cnt[2 * ci];
+        for (uint32_t i0 = 0; i0 < n_near; i0 += NF_THREADS) {
+          const uint32_t i = i0 + tid;
+          const bool valid = i < n_near;
+          int du = 0, e = 0, e_end = 0;
+          if (valid) {
+            const int u = (int)qa[i];
+            du = (int)(__hip_atomic_fetch_and(W + u, ~ST_NQ, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) >> 2);
+            e = row_offset[u];
+            e_end = row_offset[u + 1];
+          }
+          while (__any(valid && e < e_end)) {
+            bool pn = false, pf = false;
+            uint32_t item = 0;
+            if (valid && e < e_end) {
+              const int2 ed = edges[e];
+              e++;
+              const int nd = du + ed.y;
+              gu32 *wp = W + ed.x;
+              uint32_t cur = *wp;
+              while (nd < (int)(cur >> 2)) {
+                const uint32_t st = cur & 3u;
+                const bool near = nd < thr;
+                const uint32_t nw = ((uint32_t)nd << 2) | (near ? ST_NQ : (ST_NF | (st & ST_NQ)));
+                uint32_t seen = cur;  // replaced by the word found when the swap fails
+                if (__hip_atomic_compare_exchange_strong(wp, &seen, nw, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                  item = (uint32_t)ed.x;
+                  if (near) pn = !(st & ST_NQ);
+                  else pf = !(st & ST_NF);
+                  break;
+                }
+                cur = seen;
+              }
             }
+            wave_push(pn, item, qb, &cn[0]);
+            wave_push(pf, item, fa + n_far, &cn[1]);
+          }
+        }
+        const int cz = ci == 2 ? 0 : ci + 1;
+        if (tid == 0) {
+          s_cnt[2 * cz] = 0;
+          s_cnt[2 * cz + 1] = 0;
+        }
+        __syncthreads();
+        n_near = cn[0];
+        n_far += cn[1];
+        ci = cz;
+        uint32_t *t = qa;
+        qa = qb;
+        qb = t;
+        if (BIN && n_near >= bin_min) {
+          bin_queue<NF_THREADS>(qa, qb, n_near, bsh, s_hist, s_wsum);
+          t = qa;
+          qa = qb;
+          qb = t;
+        }
+      }
+      if (n_far == 0) break;
+      // ---- threshold = min live far distance + delta ----
+      if (tid == 0) s_min = 0x7FFFFFFF;
+      __syncthreads();
+      int lmin = 0x7FFFFFFF;
+      for (uint32_t i = tid; i < n_far; i += NF_THREADS) {
+        const uint32_t w = W[fa[i]];
+        if (w & ST_NF) lmin = min(lmin, (int)(w >> 2));
+      }
+      for (int off = 32; off > 0; off >>= 1) lmin = min(lmin, __shfl_down(lmin, off));
+      if ((tid & 63) == 0) atomicMin(&s_min, lmin);
+      __syncthreads();
+      const int fmin = s_min;
+      if (fmin == 0x7FFFFFFF) break;  // every far entry was stale
+      thr = fmin + delta;
+      // ---- split the far pile: below thr -> near, else stays far ----
+      uint32_t *cn = &s_cnt[2 * ci];
+      for (uint32_t i0 = 0; i0 < n_far; i0 += NF_THREADS) {
+        const uint32_t i = i0 + tid;
+        bool pn = false, pf = false;
+        uint32_t item = 0;
+        if (i < n_far) {
+          const int v = (int)fa[i];
+          item = (uint32_t)v;
+          const uint32_t w = W[v];
+          if (w & ST_NF) {
+            pn = (int)(w >> 2) < thr;
+            pf = !pn;
+            if (pn) W[v] = (w & ~3u) | ST_NQ;
           }
         }
         wave_push(pn, item, qa, &cn[0]);
@@ -267,7 +446,10 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
         qb = t;
       }
     }
-    // far piles are reused by the next source: tags make stale marks harmless
+    // every exit follows a barrier, with NQ and NF clear everywhere (no queue
+    // or live far entry is left); shift the distances back in place, each
+    // thread the words it initialised
+    for (int v = tid; v < V; v += NF_THREADS) W[v] >>= 2;
   }
 }
 
@@ -537,37 +719,49 @@ void launch_group(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t
 template <int NT>
 void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t *src,
                    int n_src, int32_t *out, int64_t ld, int32_t *const *rowptr, uint32_t *ws,
-                   int64_t stride, int delta, int bin) {
+                   int64_t stride, int delta, int bin, int pack, int64_t q_off) {
   // blocks of 2^bsh vertices, at least 64 (4 lines of distances), at most
   // NBINS blocks; queues of at least bin_min entries are binned
   int bsh = 6;
   while (((int64_t)g->V >> bsh) >= fapk::NBINS) bsh++;
   const uint32_t bin_min = bin >= 3 ? NT / 2 : bin == 2 ? NT : 2 * NT;
-  if (bin)
-    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true>), dim3(grid), dim3(NT), 0, s,
+  if (pack == 2 && bin)
+    hipLaunchKernelGGL((fapk::sssp_nfs_kernel<NT, true>), dim3(grid), dim3(NT), 0, s,
                        g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
                        delta, bsh, bin_min);
+  else if (pack == 2)
+    hipLaunchKernelGGL((fapk::sssp_nfs_kernel<NT, false>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       delta, bsh, bin_min);
+  else if (bin)
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       q_off, delta, bsh, bin_min);
   else
     hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, false>), dim3(grid), dim3(NT), 0, s,
                        g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
-                       delta, bsh, bin_min);
+                       q_off, delta, bsh, bin_min);
 }
 
 }  // namespace
 
 // Launch shape of an SSSP over n_src sources: settings, per-workgroup state
-// (6 V words for single sources, 8 V + S V for groups of S, rows of 64
+// (4 or 6 V words for single sources, 8 V + S V for groups of S, rows of 64
 // words) and grid (as many workgroups as source groups, up to 4 / 2 / 1 per
-// CU for 256 / 512 / 1024 threads, and an 8 GiB state budget).
+// CU for 256 / 512 / 1024 threads unless option sssp_wgs sets the count, and
+// an 8 GiB state budget).
 int sssp_layout(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, int *threads,
                 int *delta, int *group, int64_t *stride, int64_t *d_off) {
   sssp_settings(ctx, g, n_src, threads, delta, group);
   const int64_t vw = ((int64_t)g->V + 63) & ~(int64_t)63;
   *d_off = 8 * vw;
-  *stride = *group == 1 ? 6 * vw : *d_off + (int64_t)*group * vw;
+  // single sources: four queues, plus V 8-byte state words unless the state
+  // lives in the output rows (sssp_pack 2)
+  const int pack = ctx ? ctx->opt_sssp_pack : 1;
+  *stride = *group == 1 ? (pack == 2 ? 4 : 6) * vw : *d_off + (int64_t)*group * vw;
   const int64_t units = (n_src + *group - 1) / *group;
   const int64_t by_mem = std::max<int64_t>(1, ((int64_t)8 << 30) / (*stride * 4));
-  const int per_cu = *threads == 256 ? 4 : *threads == 512 ? 2 : 1;
+  const int per_cu = (ctx && ctx->opt_sssp_wgs > 0) ? ctx->opt_sssp_wgs : 1024 / *threads;
   return (int)std::max<int64_t>(
       1, std::min<int64_t>({units, (int64_t)ctx->num_cus * per_cu, by_mem}));
 }
@@ -596,10 +790,13 @@ int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources
   uint32_t *ws = (uint32_t *)ctx->ws_queue.ptr;
   const int32_t *src = d_sources;
   if (group == 1) {
+    // state words first (V of 8 bytes), then the four queues
     const int bin = ctx->opt_sssp_bin;
-    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
-    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
-    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin);
+    const int pack = ctx->opt_sssp_pack;
+    const int64_t q_off = 2 * (((int64_t)V + 63) & ~(int64_t)63);
+    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
+    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
+    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
   } else if (group == 4) {
     if (threads == 256) launch_group<4, 256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
     else if (threads == 512) launch_group<4, 512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);

@@ -74,6 +74,11 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "sssp_bin"     0..3     reorder SSSP near queues by vertex block when they hold at
  *                           least 2x / 1x / 0.5x the workgroup's threads (1 / 2 / 3);
  *                           0 off [3]
+ *   "sssp_pack"    0/1/2    SSSP per-vertex state: distance and queue tag in one 64-bit
+ *                           workspace word (1), or distance and two queue bits in one
+ *                           32-bit word of the output row, with a third less
+ *                           workspace (2) [1]
+ *   "sssp_wgs"     0..8     SSSP workgroups per CU [1024 threads per CU]
  *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);

@@ -173,23 +173,32 @@ def test_sssp_golden_all_sources(gpu_ctx, name):
 
 @pytest.fixture
 def sssp_config(gpu_ctx, request):
-    threads, delta, group = request.param
+    # (threads, delta, group[, pack]); pack: 1 packed 64-bit words (default),
+    # 2 in-row 32-bit words
+    threads, delta, group = request.param[:3]
+    pack = request.param[3] if len(request.param) > 3 else 1
     gpu_ctx.set_option("sssp_threads", threads)
     gpu_ctx.set_option("sssp_delta", delta)
     gpu_ctx.set_option("sssp_group", group)
+    gpu_ctx.set_option("sssp_pack", pack)
     yield request.param
     gpu_ctx.set_option("sssp_threads", 0)
     gpu_ctx.set_option("sssp_delta", 0)
     gpu_ctx.set_option("sssp_group", 0)
+    gpu_ctx.set_option("sssp_pack", 0)
+
+
+def _cfg_id(c):
+    return f"t{c[0]}-d{c[1]}-g{c[2]}" + (f"-p{c[3]}" if len(c) > 3 else "")
 
 
 SSSP_CONFIGS = [(1024, 0, 1), (256, 1, 1), (512, 7, 1), (1024, 100000, 1),
+                (1024, 0, 1, 2), (256, 1, 1, 2), (512, 7, 1, 2), (1024, 100000, 1, 2),
                 (1024, 0, 8), (256, 1, 8), (512, 7, 4), (1024, 100000, 4), (256, 0, 4),
                 (512, 0, 8)]
 
 
-@pytest.mark.parametrize("sssp_config", SSSP_CONFIGS, indirect=True,
-                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
+@pytest.mark.parametrize("sssp_config", SSSP_CONFIGS, indirect=True, ids=_cfg_id)
 def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     u, v, w = synth.grid_graph(64, 64, 0.05, 0.02, 1, 60, 1)
     V = 4096
@@ -206,11 +215,13 @@ def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     dg.close()
 
 
+@pytest.mark.parametrize("pack", [1, 2])
 @pytest.mark.parametrize("threads,delta,binq", [(256, 0, 1), (256, 1000000, 3), (1024, 0, 2),
                                                 (512, 40, 3), (1024, 0, 0)])
-def test_sssp_binned_queues(gpu_ctx, threads, delta, binq):
+def test_sssp_binned_queues(gpu_ctx, threads, delta, binq, pack):
     """Queues reordered by vertex block (option sssp_bin, each threshold, and
-    off) on a 200x200 grid, where near queues exceed the thresholds."""
+    off) on a 200x200 grid, where near queues exceed the thresholds; both
+    state layouts (option sssp_pack)."""
     u, v, w = synth.grid_graph(200, 200, 0.05, 0.02, 1, 60, 12)
     V = 40000
     ro, col, wt = synth.to_csr(V, u, v, w, False)
@@ -221,10 +232,12 @@ def test_sssp_binned_queues(gpu_ctx, threads, delta, binq):
     gpu_ctx.set_option("sssp_threads", threads)
     gpu_ctx.set_option("sssp_delta", delta)
     gpu_ctx.set_option("sssp_bin", binq)
+    gpu_ctx.set_option("sssp_pack", pack)
     try:
         dg.sssp(dsrc.ptr, len(sources), out.ptr, V)
     finally:
         gpu_ctx.set_option("sssp_bin", 3)
+        gpu_ctx.set_option("sssp_pack", 0)
         gpu_ctx.set_option("sssp_threads", 0)
         gpu_ctx.set_option("sssp_delta", 0)
     np.testing.assert_array_equal(out.get(), O.batched_sssp(sources, ro, col, wt))
@@ -232,7 +245,7 @@ def test_sssp_binned_queues(gpu_ctx, threads, delta, binq):
 
 
 @pytest.mark.parametrize("sssp_config", [(1024, 0, 8), (256, 3, 4)], indirect=True,
-                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
+                         ids=_cfg_id)
 @pytest.mark.parametrize("name", ["grid8.npz", "rgg100.npz"])
 def test_sssp_grouped_golden(gpu_ctx, sssp_config, name):
     """Grouped sources (several per workgroup pass), against the golden rows;
@@ -250,8 +263,8 @@ def test_sssp_grouped_golden(gpu_ctx, sssp_config, name):
     dg.close()
 
 
-@pytest.mark.parametrize("sssp_config", [(1024, 0, 1), (512, 0, 8)], indirect=True,
-                         ids=lambda c: f"t{c[0]}-d{c[1]}-g{c[2]}")
+@pytest.mark.parametrize("sssp_config", [(1024, 0, 1), (256, 0, 1, 2), (512, 0, 8)],
+                         indirect=True, ids=_cfg_id)
 def test_sssp_directed_zero_weights(gpu_ctx, sssp_config):
     au, av, aw = synth.rgg_graph(500, 5, 0, 9, 5)
     ro, col, w = synth.to_csr(500, au, av, aw, True)
@@ -322,8 +335,8 @@ def test_subgraph_solve_matches_oracle(gpu_ctx, R, K):
     dg.close()
 
 
-@pytest.mark.parametrize("sssp_config", [(1024, 0, 0), (512, 0, 8)], indirect=True,
-                         ids=lambda c: f"g{c[2]}")
+@pytest.mark.parametrize("sssp_config", [(1024, 0, 0), (1024, 0, 0, 2), (512, 0, 8)],
+                         indirect=True, ids=_cfg_id)
 @pytest.mark.parametrize("R,K,pick", [(20, 4, None), (33, 7, (7, 2, 5)), (16, 1, None),
                                        (40, 12, (3, 3, 12, 1))])
 def test_subgraph_solve_batch_matches_per_part(gpu_ctx, sssp_config, R, K, pick):
