@@ -706,13 +706,13 @@ def bench_sssp_sweep(args):
         dsrc = ctx.to_device(srcs)
         out = ctx.alloc((len(srcs), V))
         ref = None
-        for threads, wgs in ((256, 2), (256, 4), (256, 8), (512, 1), (512, 2), (512, 4),
-                             (1024, 1), (1024, 2)):
-            for pack, binq in ((1, 3), (1, 0), (2, 3)):
+        for threads, wgs in ((256, 4), (256, 8), (512, 2), (1024, 1)):
+            for pack, binq, arcs in ((1, 3, 1), (1, 3, 2), (1, 3, 4), (1, 0, 2), (2, 3, 2)):
                 ctx.set_option("sssp_threads", threads)
                 ctx.set_option("sssp_wgs", wgs)
                 ctx.set_option("sssp_pack", pack)
                 ctx.set_option("sssp_bin", binq)
+                ctx.set_option("sssp_arcs", arcs)
                 ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V,
                                                         ctx.stream), reps=1)
                 cs = ctx.checksum(out.ptr, len(srcs) * V)
@@ -720,6 +720,7 @@ def bench_sssp_sweep(args):
                 byts = 4.0 * (V + 1) + 8.0 * len(col) + 4.0 * len(srcs) * V
                 r = {"graph": f"{R}x{R} w1..100", "parts": parts, "sources": len(srcs),
                      "threads": threads, "wgs_per_cu": wgs, "pack": pack, "bin": binq,
+                     "arcs": arcs,
                      "ms": round(ms, 3),
                      "edges_per_s": len(srcs) * len(col) / (ms * 1e-3),
                      "algorithmic_GB_per_s": round(byts / (ms * 1e-3) / 1e9, 1),
@@ -730,6 +731,7 @@ def bench_sssp_sweep(args):
     ctx.set_option("sssp_threads", 0)
     ctx.set_option("sssp_wgs", 0)
     ctx.set_option("sssp_pack", 0)
+    ctx.set_option("sssp_arcs", 0)
     ctx.set_option("sssp_bin", 3)
     return 0
 

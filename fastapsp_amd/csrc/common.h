@@ -98,6 +98,7 @@ struct fap_ctx {
   int opt_sssp_bin = 3;          // reorder SSSP queues by vertex block from 2NT / NT / NT/2 entries (1/2/3); 0 off
   int opt_sssp_pack = 1;         // SSSP state: 64-bit words (1), in-row 32-bit words (2)
   int opt_sssp_wgs = 0;          // SSSP workgroups per CU: 1..8; 0 = 1024 threads per CU
+  int opt_sssp_arcs = 2;         // arcs per lane per SSSP relaxation step: 1, 2, 4
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse
   // once ev_stage (recorded after the last upload from it) has completed
@@ -134,5 +135,17 @@ int release_host_stage(fap_ctx *ctx, hipStream_t s);
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);
 }  // namespace fapi
+
+namespace fapk {
+// Map a linear workgroup id to a work index so that each XCD (workgroups are
+// dealt round-robin over the MI355X's 8 XCDs) takes a contiguous range of
+// work, whose shared inputs then meet in that XCD's L2.  A bijection on
+// [0, total): ids past the last multiple of 8 map to themselves.
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
+  const int per = total >> 3;
+  if (bid >= (per << 3)) return bid;
+  return (bid & 7) * per + (bid >> 3);
+}
+}  // namespace fapk
 
 #endif  // FAP_COMMON_H_

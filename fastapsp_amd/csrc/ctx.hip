@@ -245,6 +245,10 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "sssp_wgs") {
     if (value < 0 || value > 8) return fail(FAP_E_ARG, "sssp_wgs must be 0 (auto) .. 8");
     ctx->opt_sssp_wgs = (int)value;
+  } else if (k == "sssp_arcs") {
+    if (value != 0 && value != 1 && value != 2 && value != 4)
+      return fail(FAP_E_ARG, "sssp_arcs must be 0 (auto), 1, 2 or 4");
+    ctx->opt_sssp_arcs = value == 0 ? 2 : (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
     ctx->opt_batch_grid = (int)value;
@@ -266,6 +270,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_bin") *value = ctx->opt_sssp_bin;
   else if (k == "sssp_pack") *value = ctx->opt_sssp_pack;
   else if (k == "sssp_wgs") *value = ctx->opt_sssp_wgs;
+  else if (k == "sssp_arcs") *value = ctx->opt_sssp_arcs;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

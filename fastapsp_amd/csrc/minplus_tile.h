@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
+
 namespace fapk {
 
 constexpr int kInf = 0x3FFFFFFF;
@@ -205,15 +207,6 @@ using MpBig = MpTile<8, 8>;    // 128 x 128
 using MpSmall = MpTile<4, 4>;  // 64 x 64
 constexpr int MP_BM = MpBig::BM;
 constexpr int MP_BN = MpBig::BN;
-
-// Map a linear workgroup id to a tile index so that each XCD (workgroups are
-// dealt round-robin over 8 XCDs) works on a contiguous range of tiles, which
-// then share operand panels in that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int total) {
-  const int per = total >> 3;
-  if (bid >= (per << 3)) return bid;
-  return (bid & 7) * per + (bid >> 3);
-}
 
 }  // namespace fapk
 

@@ -135,7 +135,7 @@ __device__ __forceinline__ unsigned long long nf_word(int d, uint32_t tag) {
 
 __device__ __forceinline__ int nf_dist(unsigned long long w) { return (int)(w >> 32); }
 
-template <int NF_THREADS, bool BIN>
+template <int NF_THREADS, bool BIN, int AB>
 __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
     const int32_t *__restrict__ row_offset, const int2 *__restrict__ edges, int V,
     const int32_t *__restrict__ sources, int S, int32_t *out, int64_t ld,
@@ -177,31 +177,46 @@ __global__ __launch_bounds__(NF_THREADS) void sssp_nf_kernel(
             e = row_offset[u];
             e_end = row_offset[u + 1];
           }
+          // AB arcs per lane per step: their arc and state loads are issued
+          // together, so one step waits for two memory round trips, not 2 AB
+          // (AB = 2 measured 4% faster than 1 and 2% faster than 4)
           while (__any(valid && e < e_end)) {
-            bool pn = false, pf = false;
-            uint32_t item = 0;
-            if (valid && e < e_end) {
-              const int2 ed = edges[e];
-              e++;
-              const int nd = du + ed.y;
-              unsigned long long *wp = W + ed.x;
-              unsigned long long cur = *wp;
-              const bool near = nd < thr;
-              const unsigned long long nw = nf_word(nd, near ? R : ft);
-              while (nd < nf_dist(cur)) {
-                const unsigned long long prev = atomicCAS(wp, cur, nw);
-                if (prev == cur) {
-                  const uint32_t old_tag = (uint32_t)cur;
-                  item = (uint32_t)ed.x;
-                  if (near) pn = old_tag != R;
-                  else pf = old_tag != ft;
-                  break;
-                }
-                cur = prev;
-              }
+            int2 ed[AB];
+            unsigned long long cur[AB];
+#pragma unroll
+            for (int k = 0; k < AB; k++) {
+              ed[k] = make_int2(0, 0);
+              if (valid && e + k < e_end) ed[k] = edges[e + k];
             }
-            wave_push(pn, item, qb, &cn[0]);
-            wave_push(pf, item, fa + n_far, &cn[1]);
+#pragma unroll
+            for (int k = 0; k < AB; k++) {
+              cur[k] = 0;
+              if (valid && e + k < e_end) cur[k] = W[ed[k].x];
+            }
+#pragma unroll
+            for (int k = 0; k < AB; k++) {
+              bool pn = false, pf = false;
+              if (valid && e + k < e_end) {
+                const int nd = du + ed[k].y;
+                unsigned long long *wp = W + ed[k].x;
+                unsigned long long c = cur[k];
+                const bool near = nd < thr;
+                const unsigned long long nw = nf_word(nd, near ? R : ft);
+                while (nd < nf_dist(c)) {
+                  const unsigned long long prev = atomicCAS(wp, c, nw);
+                  if (prev == c) {
+                    const uint32_t old_tag = (uint32_t)c;
+                    if (near) pn = old_tag != R;
+                    else pf = old_tag != ft;
+                    break;
+                  }
+                  c = prev;
+                }
+              }
+              wave_push(pn, (uint32_t)ed[k].x, qb, &cn[0]);
+              wave_push(pf, (uint32_t)ed[k].x, fa + n_far, &cn[1]);
+            }
+            e += AB;
           }
         }
         const int cz = ci == 2 ? 0 : ci + 1;
@@ -719,7 +734,7 @@ void launch_group(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t
 template <int NT>
 void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_t *src,
                    int n_src, int32_t *out, int64_t ld, int32_t *const *rowptr, uint32_t *ws,
-                   int64_t stride, int delta, int bin, int pack, int64_t q_off) {
+                   int64_t stride, int delta, int bin, int pack, int64_t q_off, int arcs) {
   // blocks of 2^bsh vertices, at least 64 (4 lines of distances), at most
   // NBINS blocks; queues of at least bin_min entries are binned
   int bsh = 6;
@@ -733,12 +748,20 @@ void launch_single(int grid, hipStream_t s, const fap_graph_dev *g, const int32_
     hipLaunchKernelGGL((fapk::sssp_nfs_kernel<NT, false>), dim3(grid), dim3(NT), 0, s,
                        g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
                        delta, bsh, bin_min);
+  else if (bin && arcs == 4)
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true, 4>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       q_off, delta, bsh, bin_min);
+  else if (bin && arcs == 2)
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true, 2>), dim3(grid), dim3(NT), 0, s,
+                       g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
+                       q_off, delta, bsh, bin_min);
   else if (bin)
-    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true>), dim3(grid), dim3(NT), 0, s,
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, true, 1>), dim3(grid), dim3(NT), 0, s,
                        g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
                        q_off, delta, bsh, bin_min);
   else
-    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, false>), dim3(grid), dim3(NT), 0, s,
+    hipLaunchKernelGGL((fapk::sssp_nf_kernel<NT, false, 2>), dim3(grid), dim3(NT), 0, s,
                        g->row_offset, g->edges, g->V, src, n_src, out, ld, rowptr, ws, stride,
                        q_off, delta, bsh, bin_min);
 }
@@ -793,10 +816,11 @@ int run_sssp_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_sources
     // state words first (V of 8 bytes), then the four queues
     const int bin = ctx->opt_sssp_bin;
     const int pack = ctx->opt_sssp_pack;
+    const int arcs = ctx->opt_sssp_arcs;
     const int64_t q_off = 2 * (((int64_t)V + 63) & ~(int64_t)63);
-    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
-    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
-    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off);
+    if (threads == 256) launch_single<256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off, arcs);
+    else if (threads == 512) launch_single<512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off, arcs);
+    else launch_single<1024>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, delta, bin, pack, q_off, arcs);
   } else if (group == 4) {
     if (threads == 256) launch_group<4, 256>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);
     else if (threads == 512) launch_group<4, 512>(grid, s, g, src, n_src, d_out, ld_out, d_rowptr, ws, stride, d_off, delta);

@@ -262,6 +262,7 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
   peer->opt_sssp_bin = ctx->opt_sssp_bin;
   peer->opt_sssp_pack = ctx->opt_sssp_pack;
   peer->opt_sssp_wgs = ctx->opt_sssp_wgs;
+  peer->opt_sssp_arcs = ctx->opt_sssp_arcs;
   peer->opt_batch_grid = ctx->opt_batch_grid;
   peer->tune = ctx->tune;
   fap_ctx *c[2] = {ctx, peer};

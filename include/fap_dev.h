@@ -79,6 +79,7 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *                           32-bit word of the output row, with a third less
  *                           workspace (2) [1]
  *   "sssp_wgs"     0..8     SSSP workgroups per CU [1024 threads per CU]
+ *   "sssp_arcs"    0/1/2/4  arcs each lane relaxes per SSSP step, loads issued together [2]
  *   "batch_grid"   0..16    batched-closure workgroups per CU per launch [4] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);

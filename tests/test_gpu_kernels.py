@@ -215,13 +215,14 @@ def test_sssp_grid_sources_ld_and_duplicates(gpu_ctx, sssp_config):
     dg.close()
 
 
-@pytest.mark.parametrize("pack", [1, 2])
+@pytest.mark.parametrize("pack,arcs", [(1, 4), (1, 2), (1, 1), (2, 4)])
 @pytest.mark.parametrize("threads,delta,binq", [(256, 0, 1), (256, 1000000, 3), (1024, 0, 2),
                                                 (512, 40, 3), (1024, 0, 0)])
-def test_sssp_binned_queues(gpu_ctx, threads, delta, binq, pack):
+def test_sssp_binned_queues(gpu_ctx, threads, delta, binq, pack, arcs):
     """Queues reordered by vertex block (option sssp_bin, each threshold, and
     off) on a 200x200 grid, where near queues exceed the thresholds; both
-    state layouts (option sssp_pack)."""
+    state layouts (option sssp_pack) and arcs per relaxation step
+    (option sssp_arcs)."""
     u, v, w = synth.grid_graph(200, 200, 0.05, 0.02, 1, 60, 12)
     V = 40000
     ro, col, wt = synth.to_csr(V, u, v, w, False)
@@ -233,11 +234,13 @@ def test_sssp_binned_queues(gpu_ctx, threads, delta, binq, pack):
     gpu_ctx.set_option("sssp_delta", delta)
     gpu_ctx.set_option("sssp_bin", binq)
     gpu_ctx.set_option("sssp_pack", pack)
+    gpu_ctx.set_option("sssp_arcs", arcs)
     try:
         dg.sssp(dsrc.ptr, len(sources), out.ptr, V)
     finally:
         gpu_ctx.set_option("sssp_bin", 3)
         gpu_ctx.set_option("sssp_pack", 0)
+        gpu_ctx.set_option("sssp_arcs", 0)
         gpu_ctx.set_option("sssp_threads", 0)
         gpu_ctx.set_option("sssp_delta", 0)
     np.testing.assert_array_equal(out.get(), O.batched_sssp(sources, ro, col, wt))
@@ -265,14 +268,19 @@ def test_sssp_grouped_golden(gpu_ctx, sssp_config, name):
 
 @pytest.mark.parametrize("sssp_config", [(1024, 0, 1), (256, 0, 1, 2), (512, 0, 8)],
                          indirect=True, ids=_cfg_id)
-def test_sssp_directed_zero_weights(gpu_ctx, sssp_config):
+@pytest.mark.parametrize("arcs", [4, 1])
+def test_sssp_directed_zero_weights(gpu_ctx, sssp_config, arcs):
     au, av, aw = synth.rgg_graph(500, 5, 0, 9, 5)
     ro, col, w = synth.to_csr(500, au, av, aw, True)
     dg = gpu_ctx.upload_graph(ro, col, w)
     sources = np.arange(0, 500, 3, dtype=np.int32)
     dsrc = gpu_ctx.to_device(sources)
     out = gpu_ctx.alloc((len(sources), 500))
-    dg.sssp(dsrc.ptr, len(sources), out.ptr, 500)
+    gpu_ctx.set_option("sssp_arcs", arcs)
+    try:
+        dg.sssp(dsrc.ptr, len(sources), out.ptr, 500)
+    finally:
+        gpu_ctx.set_option("sssp_arcs", 0)
     np.testing.assert_array_equal(out.get(), O.batched_sssp(sources, ro, col, w))
     dg.close()
 
