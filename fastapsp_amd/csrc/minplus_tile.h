@@ -138,8 +138,12 @@ struct MpTile {
     for (int ch = 0; ch < nchunks; ch++) {
       const bool more = ch + 1 < nchunks;
       if (more) load_chunk(A, lda, B, ldb, m_valid, n_valid, K, (ch + 1) * MP_BK, flags, ra, rb);
+      // the last chunk stops after its last valid k pair (K = 115 computes
+      // 116 k, not 128); a k past K reads INF from LDS and changes nothing
+      const int krem = K - ch * MP_BK;
+      const int kend = krem >= MP_BK ? MP_BK : (krem + 1) & ~1;
 #pragma unroll 1
-      for (int kk = 0; kk < MP_BK; kk += 2) {
+      for (int kk = 0; kk < kend; kk += 2) {
         int a0[TM], a1[TM], b0[TN], b1[TN];
 #pragma unroll
         for (int r = 0; r < TM; r++) {

@@ -38,7 +38,7 @@ def unpad(dev, rows, cols):
 # ---------------- min-plus ----------------
 
 MP_SHAPES = [(1, 1, 1), (5, 7, 3), (128, 128, 16), (129, 257, 17), (64, 300, 2), (300, 1000, 124),
-             (257, 129, 250), (33, 4099, 61)]
+             (257, 129, 250), (33, 4099, 61), (200, 300, 115), (70, 140, 15), (9, 600, 31)]
 
 
 @pytest.mark.parametrize("M,N,K", MP_SHAPES)
