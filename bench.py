@@ -13,8 +13,13 @@ fap/kernel/cuda/minplus.cu:483).
 
 The roofline object is for the min-plus product kernel, timed with HIP events
 on its own stream over the timed steps; the Floyd-Warshall closure is
-reported beside it.  Peak = VALU int32: 256 CUs x 64 lanes x 2.4 GHz =
-39.3 T int32 ops/s (min-plus is not a multiply-accumulate, so no MFMA).
+reported beside it.  Min-plus is not a multiply-accumulate, so the bound is
+the VALU, not MFMA.  peak = the measured ceiling of the kernels' instruction
+mix (one v_add_u32 per update, one v_min3_i32 per two), probed on the same
+device in the same run (fap_probe_minplus_ceiling); on gfx950 v_add_u32
+issues at twice the rate of v_min3_i32, so that ceiling (~76 TOP/s) is
+above the spec figure of 1 int32 op per lane per clock, 256 CUs x 64 lanes x
+2.4 GHz = 39.3 T, which is reported beside it as spec_peak.
 
 cpu_baseline: the reference's own CPU code (oracle/_ref, compiled from its
 headers; min_plus_path_CPU_advanced, single-threaded like the reference) on
@@ -38,7 +43,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.32 T int32 ops/s
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # spec: 1 int32 op per lane per clock, 39.32 T
 # HBM bytes per launch of the n = 8192 min-plus product, from two rocprofv3
 # counter runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a copy of
 # known size): 6.66e9 read + 0.27e9 written.
@@ -90,6 +95,8 @@ def bench_dense(args):
     B = synth.dense_matrix(n, 0.01, 1, 1000, 22)
     log(f"[bench] generated in {time.time() - t0:.1f}s")
     ctx = fa.Context(0)
+    ceiling = ctx.probe_minplus_ceiling()
+    log(f"[bench] min-plus instruction-mix ceiling {ceiling:.2f} TOP/s")
     tuned = _tune(ctx, args, "fw", fa.TuneCache.key_fw(n), lambda: ctx.tune_fw(n, reps=1))
     d_src = ctx.to_device(M0)
     d_fw = ctx.alloc((n, n))
@@ -153,13 +160,18 @@ def bench_dense(args):
         "config": {"workload": f"dense FW + min-plus product, n={n}, arc p=0.01, w 1..1000",
                    "n": n, "ops_per_step": ops_step},
         "roofline": {"bound": "valu", "kernel": "minplus_gemm_kernel",
-                     "achieved": round(ach, 2), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(ach / VALU_PEAK_TOPS, 3),
+                     "achieved": round(ach, 2), "peak": round(ceiling, 2),
+                     "unit": "TOP/s", "frac": round(ach / ceiling, 3),
                      "traffic": MINPLUS_8192_TRAFFIC if n == 8192 else None,
+                     "peak_kind": "measured v_add_u32 + v_min3_i32 mix ceiling",
+                     "spec_peak": round(VALU_PEAK_TOPS, 2),
+                     "frac_of_spec_peak": round(ach / VALU_PEAK_TOPS, 3),
                      "ms_per_launch": round(mp_avg, 3)},
         "roofline_fw": {"bound": "valu", "kernels": "fw_diag/fw_panel/fw_rest",
-                        "achieved": round(ach_fw, 2), "peak": round(VALU_PEAK_TOPS, 2),
-                        "unit": "TOP/s", "frac": round(ach_fw / VALU_PEAK_TOPS, 3),
+                        "achieved": round(ach_fw, 2), "peak": round(ceiling, 2),
+                        "unit": "TOP/s", "frac": round(ach_fw / ceiling, 3),
+                        "spec_peak": round(VALU_PEAK_TOPS, 2),
+                        "frac_of_spec_peak": round(ach_fw / VALU_PEAK_TOPS, 3),
                         "ms_per_closure": round(fw_avg, 3)},
         "cpu_baseline": cpu,
         "tuned": tuned,
@@ -308,7 +320,7 @@ def bench_grid_pipeline(args):
                      "minplus": round(t_mp, 2), "scatter": round(t_glue, 2),
                      "wall_incl_checksum": round(total * 1e3, 2)},
         "minplus_stage": {"ops": ops_mp, "TOP/s": round(ops_mp / (t_mp * 1e-3) / 1e12, 2),
-                          "frac_valu_peak": round(ops_mp / (t_mp * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
+                          "frac_of_spec_peak": round(ops_mp / (t_mp * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
         "fw_stage": {"ops": ops_fw, "TOP/s": round(ops_fw / (t_fw * 1e-3) / 1e12, 2)},
         "sssp_stage": {"sources": srcs, "edges_per_s": srcs * E / (t_sssp * 1e-3),
                        "algorithmic_GB_per_s": round(sssp_bytes / (t_sssp * 1e-3) / 1e9, 1)},
@@ -747,6 +759,7 @@ def bench_batch(args):
     from fastapsp_amd import synth
     lib = fa.load()
     ctx = fa.Context(0)
+    ceiling = ctx.probe_minplus_ceiling()
     sizes = synth.loguniform_sizes(args.jobs, 64, 2048, 4).astype(int)
     graphs, ptrs, lds = [], [], []
     ld_of = [(n + 3) // 4 * 4 for n in sizes]
@@ -805,7 +818,9 @@ def bench_batch(args):
                   "GOp/s": round(ops / (res["plain"] * 1e-3) / 1e9, 1)},
         "batched": {"ms": round(res["batched"], 3),
                     "GOp/s": round(ops / (res["batched"] * 1e-3) / 1e9, 1),
-                    "frac_valu_peak": round(ops / (res["batched"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
+                    "frac_of_ceiling": round(ops / (res["batched"] * 1e-3) / 1e12 / ceiling, 3),
+                    "frac_of_spec_peak": round(ops / (res["batched"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
+        "minplus_ceiling_TOPs": round(ceiling, 2),
         "batched_launches": info["launches"], "batched_tasks": info["tasks"],
         "speedup_batched_vs_plain": round(res["plain"] / res["batched"], 2),
         "plain_and_batched_identical": bool(same),

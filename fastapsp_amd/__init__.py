@@ -175,6 +175,7 @@ _SIGS = {
     "fap_tune_fw": (_int, [_vp, _i32, _i32]),
     "fap_tune_sssp": (_int, [_vp, _vp, _pi32, _i32, _i32]),
     "fap_tune_batch": (_int, [_vp, _i32, _pi32, _i32]),
+    "fap_probe_minplus_ceiling": (_int, [_vp, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -373,6 +374,14 @@ class Context:
     def tune_batch(self, sizes, reps: int = 2):
         n = np.ascontiguousarray(sizes, dtype=np.int32)
         check(load().fap_tune_batch(self.handle, n.size, _p32(n), int(reps)), "tune_batch")
+
+    def probe_minplus_ceiling(self):
+        """Measured TOP/s of the min-plus instruction mix (2 ops per update)
+        on this device: the VALU ceiling the min-plus and FW kernels run
+        against."""
+        v = C.c_double()
+        check(load().fap_probe_minplus_ceiling(self.handle, C.byref(v)), "probe_minplus_ceiling")
+        return v.value
 
     def alloc(self, shape) -> DeviceArray:
         shape = (shape,) if isinstance(shape, (int, np.integer)) else tuple(shape)

@@ -72,9 +72,16 @@ __device__ __forceinline__ void batch_diag(int32_t *blk, int64_t ld, int kb, int
   }
 }
 
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_phase_kernel(
+// KIND >= 0: every task of the launch has that kind (the host checks), so
+// the other paths compile out and the kernel needs fewer registers (the
+// remaining-tile instance runs 6 waves per SIMD instead of 4); KIND = -1
+// handles launches that mix kinds (staggered jobs).
+template <int KIND>
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND > 0 ? 6 : 4))) void fw_phase_kernel(
     const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, int nseg, int ntasks) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[BATCH_LDS_WORDS];
+  constexpr int LDS_WORDS = (KIND == TASK_PANEL || KIND == TASK_REST) ? MpSmall::LDS_WORDS
+                                                                       : BATCH_LDS_WORDS;
+  __shared__ __attribute__((aligned(16))) int32_t lds[LDS_WORDS];
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     // the segment holding task ti: segments tile [0, ntasks) in order
     int lo = 0, hi = nseg - 1;
@@ -85,7 +92,8 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     }
     const BatchSeg sg = segs[lo];
     const BatchJob job = jobs[sg.job];
-    const int kind = sg.kind_r & 3, r = sg.kind_r >> 2, idx = ti - sg.first;
+    const int kind = KIND >= 0 ? KIND : (sg.kind_r & 3);
+    const int r = sg.kind_r >> 2, idx = ti - sg.first;
     const int nb = job.nb, n = job.n;
     const int64_t ld = job.ld;
     int32_t *d = job.mat;
@@ -93,9 +101,9 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     const int kb = imin(BT, n - base);
     const int flags =
         ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
-    if (kind == TASK_DIAG) {
+    if ((KIND < 0 || KIND == TASK_DIAG) && kind == TASK_DIAG) {
       batch_diag(d + (int64_t)base * ld + base, ld, kb, lds);
-    } else if (kind == TASK_PANEL) {
+    } else if ((KIND < 0 || KIND == TASK_PANEL) && kind == TASK_PANEL) {
       const int32_t *diag = d + (int64_t)base * ld + base;
       if (idx < nb - 1) {
         const int J = idx < r ? idx : idx + 1;
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
         int32_t *c = d + (int64_t)I * BT * ld + base;
         MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
       }
-    } else {
+    } else if (KIND < 0 || KIND == TASK_REST) {
       const int m = nb - 1;
       int I = idx / m, J = idx - (idx / m) * m;
       I += (I >= r);
@@ -182,8 +190,23 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
     const int cnt = desc.launch_tasks[p];
     const int nseg = desc.seg_off[p + 1] - desc.seg_off[p];
     if (cnt <= 0 || nseg <= 0) continue;
-    hipLaunchKernelGGL(fapk::fw_phase_kernel, dim3(std::min(cnt, cap)), dim3(fapk::MP_THREADS),
-                       0, s, d_jobs, d_segs + desc.seg_off[p], nseg, cnt);
+    // one task kind in the whole launch -> the specialised kernel
+    int kind = desc.segs[desc.seg_off[p]].kind_r & 3;
+    for (int q = desc.seg_off[p] + 1; q < desc.seg_off[p + 1]; q++)
+      if ((desc.segs[q].kind_r & 3) != kind) kind = -1;
+    const dim3 grid(std::min(cnt, cap)), block(fapk::MP_THREADS);
+    const fapk::BatchSeg *sg = d_segs + desc.seg_off[p];
+    if (kind == fapk::TASK_DIAG)
+      hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_DIAG>, grid, block, 0, s, d_jobs, sg,
+                         nseg, cnt);
+    else if (kind == fapk::TASK_PANEL)
+      hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_PANEL>, grid, block, 0, s, d_jobs, sg,
+                         nseg, cnt);
+    else if (kind == fapk::TASK_REST)
+      hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_REST>, grid, block, 0, s, d_jobs, sg,
+                         nseg, cnt);
+    else
+      hipLaunchKernelGGL(fapk::fw_phase_kernel<-1>, grid, block, 0, s, d_jobs, sg, nseg, cnt);
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;

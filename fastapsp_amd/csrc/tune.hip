@@ -31,6 +31,31 @@ __global__ void tune_fill_kernel(int32_t *__restrict__ d, int32_t n, int64_t ld,
   }
 }
 
+// Issue-rate probe for the min-plus instruction mix: per update one
+// v_add_u32, per two updates one v_min3_i32 folding both sums into the
+// accumulator -- the inner loop of the min-plus tile without its LDS reads.
+// Eight independent accumulators per lane; inline asm keeps the compiler
+// from folding the chains.  Each thread writes only its own output word.
+__global__ __launch_bounds__(256) void valu_mix_kernel(int32_t *out, int iters, int seed) {
+  int a0 = threadIdx.x + seed, a1 = a0 ^ 1, a2 = a0 ^ 2, a3 = a0 ^ 3;
+  int a4 = a0 ^ 4, a5 = a0 ^ 5, a6 = a0 ^ 6, a7 = a0 ^ 7;
+  const int b = seed * 3 + 1, c = seed * 5 + 2;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int t0, t1;
+#define FAP_MIX(acc, x, y)                                                           \
+  asm volatile("v_add_u32 %0, %1, %2" : "=v"(t0) : "v"(x), "v"(b));                 \
+  asm volatile("v_add_u32 %0, %1, %2" : "=v"(t1) : "v"(y), "v"(c));                 \
+  asm volatile("v_min3_i32 %0, %0, %1, %2" : "+v"(acc) : "v"(t0), "v"(t1));
+      FAP_MIX(a0, a1, a2) FAP_MIX(a1, a2, a3) FAP_MIX(a2, a3, a4) FAP_MIX(a3, a4, a5)
+      FAP_MIX(a4, a5, a6) FAP_MIX(a5, a6, a7) FAP_MIX(a6, a7, a0) FAP_MIX(a7, a0, a1)
+#undef FAP_MIX
+    }
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
 }  // namespace fapk
 
 namespace fapi {
@@ -260,6 +285,37 @@ int fap_tune_batch(fap_ctx *ctx, int32_t count, const int32_t *n, int32_t reps) 
   ctx->opt_batch_grid = saved;
   FAP_TRY(rc);
   return fap_tune_cache_set(ctx->tune, "batch", fap_tune_key_batch(count, n), bp, 1, best);
+}
+
+int fap_probe_minplus_ceiling(fap_ctx *ctx, double *tops) {
+  if (!ctx || !tops) return fail(FAP_E_ARG, "null argument");
+  // 8 workgroups of 256 threads per CU (8 waves per SIMD), 16 updates per
+  // unrolled step and lane: 64 steps x iters, ~15 ms per launch; one
+  // warm-up launch, then the fastest of three
+  const int blocks = ctx->num_cus * 8, iters = 8000;
+  Scratch out;
+  FAP_TRY(out.alloc(sizeof(int32_t) * (size_t)blocks * 256));
+  hipStream_t s = ctx->stream;
+  Events ev;
+  FAP_TRY(ev.create());
+  hipLaunchKernelGGL(fapk::valu_mix_kernel, dim3(blocks), dim3(256), 0, s, (int32_t *)out.p,
+                     iters, 1);
+  FAP_CHECK_LAUNCH();
+  float best = 0;
+  for (int rep = 0; rep < 3; rep++) {
+    FAP_HIP(hipEventRecord(ev.a, s));
+    hipLaunchKernelGGL(fapk::valu_mix_kernel, dim3(blocks), dim3(256), 0, s, (int32_t *)out.p,
+                       iters, 2 + rep);
+    FAP_CHECK_LAUNCH();
+    FAP_HIP(hipEventRecord(ev.b, s));
+    FAP_HIP(hipEventSynchronize(ev.b));
+    float ms = 0;
+    FAP_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+    if (rep == 0 || ms < best) best = ms;
+  }
+  const double updates = (double)blocks * 256 * iters * 8 * 16;
+  *tops = 2.0 * updates / (best * 1e-3) / 1e12;
+  return FAP_OK;
 }
 
 }  // extern "C"

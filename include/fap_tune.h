@@ -69,6 +69,14 @@ int fap_tune_sssp(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *h_sources
                   int32_t reps);
 int fap_tune_batch(fap_ctx *ctx, int32_t count, const int32_t *n, int32_t reps);
 
+/* Measured ceiling of the min-plus instruction mix on this device (no
+ * counterpart in the reference): one v_add_u32 per update and one
+ * v_min3_i32 per two updates, eight independent chains per lane, every CU
+ * busy, no memory traffic.  Reported in TOP/s at 2 ops per update, the
+ * unit of the min-plus and FW rates; on gfx950 v_add_u32 issues at twice
+ * the v_min3_i32 rate, so this exceeds 1 op per lane per clock. */
+int fap_probe_minplus_ceiling(fap_ctx *ctx, double *tops);
+
 #ifdef __cplusplus
 }
 #endif

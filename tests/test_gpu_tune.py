@@ -86,3 +86,11 @@ def test_batch_follows_cache_and_stays_exact(per_cu):
         ctx.fw_batched([d.ptr for d in ds], sizes, sizes)
         for d, m in zip(ds, mats):
             np.testing.assert_array_equal(d.get(), O.floyd(m))
+
+
+def test_probe_minplus_ceiling(gpu_ctx):
+    """The instruction-mix probe runs and reports a sane rate: on an idle
+    MI355X ~76 TOP/s (v_add_u32 issues at twice the rate of v_min3_i32);
+    the bounds leave room for clock changes and other work on the device."""
+    tops = gpu_ctx.probe_minplus_ceiling()
+    assert 20.0 < tops < 100.0, tops
