@@ -36,41 +36,8 @@
 namespace fapk {
 
 
-constexpr int BT_LD = BT + 4;
 constexpr int BATCH_LDS_WORDS =
-    (MpSmall::LDS_WORDS > BT * BT_LD) ? MpSmall::LDS_WORDS : BT * BT_LD;
-
-// Closure of one diagonal 64-block with 256 threads: thread t owns the
-// 1 x 4 strips at rows (t >> 4) + 16q, q < 4, columns (t & 15) * 4.
-__device__ __forceinline__ void batch_diag(int32_t *blk, int64_t ld, int kb, int32_t *D) {
-  const int t = threadIdx.x;
-  for (int idx = t; idx < BT * BT; idx += MP_THREADS) {
-    const int i = idx >> 6, j = idx & 63;
-    D[i * BT_LD + j] = (i < kb && j < kb) ? blk[(int64_t)i * ld + j] : kInf;
-  }
-  __syncthreads();
-  const int r0 = t >> 4, j = (t & 15) << 2;
-  for (int k = 0; k < kb; k++) {
-    const int4 b = *reinterpret_cast<const int4 *>(D + k * BT_LD + j);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int i = r0 + 16 * q;
-      const int a = D[i * BT_LD + k];
-      int4 *cp = reinterpret_cast<int4 *>(D + i * BT_LD + j);
-      int4 c = *cp;
-      c.x = imin(c.x, a + b.x);
-      c.y = imin(c.y, a + b.y);
-      c.z = imin(c.z, a + b.z);
-      c.w = imin(c.w, a + b.w);
-      *cp = c;
-    }
-    __syncthreads();
-  }
-  for (int idx = t; idx < BT * BT; idx += MP_THREADS) {
-    const int i = idx >> 6, j2 = idx & 63;
-    if (i < kb && j2 < kb) blk[(int64_t)i * ld + j2] = D[i * BT_LD + j2];
-  }
-}
+    (MpSmall::LDS_WORDS > BLK_LDS_WORDS) ? MpSmall::LDS_WORDS : BLK_LDS_WORDS;
 
 // KIND >= 0: every task of the launch has that kind (the host checks), so
 // the other paths compile out and the kernel needs fewer registers (the
@@ -102,7 +69,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
     const int flags =
         ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
     if ((KIND < 0 || KIND == TASK_DIAG) && kind == TASK_DIAG) {
-      batch_diag(d + (int64_t)base * ld + base, ld, kb, lds);
+      close_block64(d + (int64_t)base * ld + base, ld, kb, lds);
     } else if ((KIND < 0 || KIND == TASK_PANEL) && kind == TASK_PANEL) {
       const int32_t *diag = d + (int64_t)base * ld + base;
       if (idx < nb - 1) {

@@ -137,16 +137,28 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 // Modes 1 and 2 together equal mode 0; splitting them lets the next round's
 // diagonal and panels (which only touch row/column p) start while mode 2
 // runs.
+//
+// With `fuse` (T = 64, mode 0, r + 1 < nb) workgroup 0 takes tile
+// (r+1, r+1) -- swapping with the workgroup that had it -- and after its
+// update closes it in LDS: the next round's diagonal phase, done while the
+// other tiles of this phase run, so that round launches only its panels.
 template <int TM>
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_rest_kernel(
-    int32_t *d, int n, int64_t ld, int r, int nb, int flags, int mode) {
+    int32_t *d, int n, int64_t ld, int r, int nb, int flags, int mode, int fuse) {
   using Tile = MpTile<TM, TM>;
   constexpr int T = Tile::BM;
-  __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
+  constexpr int LDS_WORDS = (TM == 4 && BLK_LDS_WORDS > Tile::LDS_WORDS) ? BLK_LDS_WORDS
+                                                                         : Tile::LDS_WORDS;
+  __shared__ __attribute__((aligned(16))) int32_t lds[LDS_WORDS];
   int I, J;
   if (mode == 0) {
     const int m = nb - 1;
-    const int t = xcd_remap(blockIdx.x, m * m);
+    int t = xcd_remap(blockIdx.x, m * m);
+    if (fuse) {
+      const int tp = r * m + r;  // tile (r+1, r+1) in the compressed index
+      if (blockIdx.x == 0) t = tp;
+      else if (t == tp) t = xcd_remap(0, m * m);
+    }
     I = t / m;
     J = t - I * m;
     I += (I >= r);
@@ -176,6 +188,10 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   const int32_t *a = d + (int64_t)I * T * ld + base;
   const int32_t *b = d + (int64_t)base * ld + (int64_t)J * T;
   Tile::run(a, ld, b, ld, c, ld, imin(T, n - I * T), imin(T, n - J * T), kb, true, flags, lds);
+  if (TM == 4 && fuse && blockIdx.x == 0) {
+    __syncthreads();  // the tile's writes above come before the closure reads it
+    close_block64(c, ld, imin(T, n - I * T), lds);
+  }
 }
 
 }  // namespace fapk
@@ -192,10 +208,12 @@ struct Fw {
   static constexpr int T = 16 * TM;
 
   static int diag_panel(int32_t *d, int32_t n, int64_t ld, int r, int nb, int flags,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(fapk::fw_diag_kernel<T>, dim3(1), dim3(fapk::FW_DIAG_THREADS), 0, s, d,
-                       n, ld, r);
-    FAP_CHECK_LAUNCH();
+                        hipStream_t s, bool diag = true) {
+    if (diag) {
+      hipLaunchKernelGGL(fapk::fw_diag_kernel<T>, dim3(1), dim3(fapk::FW_DIAG_THREADS), 0, s,
+                         d, n, ld, r);
+      FAP_CHECK_LAUNCH();
+    }
     if (nb > 1) {
       hipLaunchKernelGGL(fapk::fw_panel_kernel<TM>, dim3(2 * (nb - 1)), dim3(fapk::MP_THREADS),
                          0, s, d, n, ld, r, nb, flags);
@@ -205,12 +223,12 @@ struct Fw {
   }
 
   static int rest(int32_t *d, int32_t n, int64_t ld, int r, int nb, int flags, int mode,
-                  hipStream_t s) {
+                  hipStream_t s, int fuse = 0) {
     const int tiles =
         mode == 0 ? (nb - 1) * (nb - 1) : mode == 1 ? 2 * nb - 3 : (nb - 2) * (nb - 2);
     if (tiles <= 0) return FAP_OK;
     hipLaunchKernelGGL(fapk::fw_rest_kernel<TM>, dim3(tiles), dim3(fapk::MP_THREADS), 0, s, d,
-                       n, ld, r, nb, flags, mode);
+                       n, ld, r, nb, flags, mode, fuse);
     FAP_CHECK_LAUNCH();
     return FAP_OK;
   }
@@ -235,8 +253,10 @@ struct Fw {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 2, s));
         FAP_HIP(hipStreamWaitEvent(s, ctx->ev_fw_panel, 0));
       } else {
-        FAP_TRY(rest(d, n, ld, r, nb, flags, 0, s));
-        if (r + 1 < nb) FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, s));
+        // T = 64: round r's phase 3 also closes the next diagonal block
+        const int fuse = (TM == 4 && r + 1 < nb) ? 1 : 0;
+        FAP_TRY(rest(d, n, ld, r, nb, flags, 0, s, fuse));
+        if (r + 1 < nb) FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, s, !fuse));
       }
     }
     return FAP_OK;

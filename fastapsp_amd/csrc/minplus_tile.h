@@ -209,6 +209,45 @@ struct MpTile {
 
 using MpBig = MpTile<8, 8>;    // 128 x 128
 using MpSmall = MpTile<4, 4>;  // 64 x 64
+
+// Floyd-Warshall closure of one diagonal block of up to 64 x 64 (kb valid
+// rows and columns; the rest read as INF) with 256 threads, in LDS (`D`
+// holds BLK_LDS_WORDS ints, row stride BLK_LD): thread t owns the 1 x 4
+// strips at rows (t >> 4) + 16q, q < 4, columns (t & 15) * 4.  Row k and
+// column k do not change during pivot k (the diagonal is >= 0), so every
+// thread updates in place; one barrier per pivot.
+constexpr int BLK_LD = 64 + 4;
+constexpr int BLK_LDS_WORDS = 64 * BLK_LD;
+
+__device__ __forceinline__ void close_block64(int32_t *blk, int64_t ld, int kb, int32_t *D) {
+  const int t = threadIdx.x;
+  for (int idx = t; idx < 64 * 64; idx += MP_THREADS) {
+    const int i = idx >> 6, j = idx & 63;
+    D[i * BLK_LD + j] = (i < kb && j < kb) ? blk[(int64_t)i * ld + j] : kInf;
+  }
+  __syncthreads();
+  const int r0 = t >> 4, j = (t & 15) << 2;
+  for (int k = 0; k < kb; k++) {
+    const int4 b = *reinterpret_cast<const int4 *>(D + k * BLK_LD + j);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = r0 + 16 * q;
+      const int a = D[i * BLK_LD + k];
+      int4 *cp = reinterpret_cast<int4 *>(D + i * BLK_LD + j);
+      int4 c = *cp;
+      c.x = imin(c.x, a + b.x);
+      c.y = imin(c.y, a + b.y);
+      c.z = imin(c.z, a + b.z);
+      c.w = imin(c.w, a + b.w);
+      *cp = c;
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < 64 * 64; idx += MP_THREADS) {
+    const int i = idx >> 6, j2 = idx & 63;
+    if (i < kb && j2 < kb) blk[(int64_t)i * ld + j2] = D[i * BLK_LD + j2];
+  }
+}
 constexpr int MP_BM = MpBig::BM;
 constexpr int MP_BN = MpBig::BN;
 
