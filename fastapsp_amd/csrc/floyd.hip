@@ -243,6 +243,8 @@ struct Fw {
     const int flags = vec_flags(d, ld, d, ld, d, ld);
     FAP_TRY(diag_panel(d, n, ld, 0, nb, flags, s));
     const bool overlap = lookahead && ctx && ctx->aux_stream && nb >= 3;
+    const bool fuse_ok =
+        TM == 4 && (int64_t)(nb - 1) * (nb - 1) >= (int64_t)(ctx ? ctx->num_cus : 256) * 8;
     for (int r = 0; r < nb; r++) {
       if (r + 1 < nb && overlap) {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 1, s));
@@ -253,8 +255,11 @@ struct Fw {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 2, s));
         FAP_HIP(hipStreamWaitEvent(s, ctx->ev_fw_panel, 0));
       } else {
-        // T = 64: round r's phase 3 also closes the next diagonal block
-        const int fuse = (TM == 4 && r + 1 < nb) ? 1 : 0;
+        // T = 64: round r's phase 3 also closes the next diagonal block when
+        // the phase is long enough to hide a 256-thread closure (at least 8
+        // tiles per CU; below that the 1024-thread diagonal kernel is
+        // faster: n = 1024 took 0.56 ms unfused, 0.64 ms fused)
+        const int fuse = (fuse_ok && r + 1 < nb) ? 1 : 0;
         FAP_TRY(rest(d, n, ld, r, nb, flags, 0, s, fuse));
         if (r + 1 < nb) FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, s, !fuse));
       }
