@@ -133,6 +133,39 @@ def test_fw_matches_oracle(gpu_ctx, fw_config, n):
     np.testing.assert_array_equal(d.get(), O.floyd(m))
 
 
+def test_fw_fused_diagonal_matches_unfused(gpu_ctx):
+    """n = 3100 (49 blocks of 64, the last one 28 wide): big enough that each
+    round's remaining-tile launch also closes the next diagonal block (at
+    least 8 tiles per CU).  The whole closure equals the unfused forms
+    (look-ahead on, and 128-wide blocks), and sampled rows equal Dijkstra
+    rows of the matrix graph."""
+    n = 3100
+    m = synth.dense_matrix(n, 0.004, 1, 1000, 31)
+    ref = None
+    for tile, look in ((64, 0), (64, 1), (128, 0)):
+        gpu_ctx.set_option("fw_tile", tile)
+        gpu_ctx.set_option("fw_lookahead", look)
+        try:
+            d = gpu_ctx.to_device(m)
+            gpu_ctx.fw(d.ptr, n, n)
+            got = d.get()
+            d.free()
+        finally:
+            gpu_ctx.set_option("fw_tile", 0)
+            gpu_ctx.set_option("fw_lookahead", -1)
+        if ref is None:
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref)
+    rows = np.array([0, 1500, 3071, 3099])
+    mask = (m < O.INF) & ~np.eye(n, dtype=bool)
+    src, dst = np.nonzero(mask)
+    ro = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(src, minlength=n), out=ro[1:])
+    want = O.batched_sssp(rows, ro, dst.astype(np.int32), m[src, dst].astype(np.int32))
+    np.testing.assert_array_equal(ref[rows], want)
+
+
 def test_fw_golden_and_strided(gpu_ctx):
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dense70.npz"))
