@@ -106,6 +106,9 @@ struct fap_ctx {
   size_t host_stage_bytes = 0;
   hipEvent_t ev_stage = nullptr;
   bool stage_pending = false;
+  // recorded after a batched part solve's SSSP launch, so that the next
+  // group's SSSP (on the peer's stream) starts after it, not beside it
+  hipEvent_t ev_sssp = nullptr;
   // second context (own stream and workspaces) for two-stream pipelines of
   // batched part solves; created on first use, owned
   fap_ctx *peer = nullptr;

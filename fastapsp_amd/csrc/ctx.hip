@@ -185,6 +185,7 @@ int fap_ctx_create(int device, fap_ctx **out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_la, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fw_panel, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_sssp, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "context setup", __FILE__, __LINE__);
@@ -206,6 +207,7 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
   if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
+  if (ctx->ev_sssp) (void)hipEventDestroy(ctx->ev_sssp);
   if (ctx->ev_poll) (void)hipEventDestroy(ctx->ev_poll);
   if (ctx->ev_fw_la) (void)hipEventDestroy(ctx->ev_fw_la);
   if (ctx->ev_fw_panel) (void)hipEventDestroy(ctx->ev_fw_panel);

@@ -168,10 +168,13 @@ int run_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int3
 // one batched closure, then each part's product and scatter.  Sources, row
 // pointers and closure descriptors go up in one copy from the context's
 // pinned staging.  ctx's workspaces stay in use until s passes this work;
-// a later enqueue on the same stream is ordered after it.
+// a later enqueue on the same stream is ordered after it.  The SSSP launch
+// waits for `sssp_after` (if given) and is followed by a record of
+// `sssp_done` (if given).
 int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
                            const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
-                           int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
+                           int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s,
+                           hipEvent_t sssp_after = nullptr, hipEvent_t sssp_done = nullptr) {
   if (count <= 0) return FAP_OK;
   if ((int64_t)g->h_st2ed.size() != g->V)
     return fail(FAP_E_STATE, "graph has no partition (fap_graph_set_partition)");
@@ -211,9 +214,11 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   char *d = (char *)ctx->ws_srcs.ptr;
   FAP_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
   FAP_TRY(release_host_stage(ctx, s));
+  if (sssp_after) FAP_HIP(hipStreamWaitEvent(s, sssp_after, 0));
   if (nsrc > 0)
     FAP_TRY(run_sssp_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, nullptr, 0,
                           (int32_t *const *)(d + o_rp), s));
+  if (sssp_done) FAP_HIP(hipEventRecord(sssp_done, s));
   for (int32_t p = 0; p < count; p++)
     FAP_TRY(launch_gather(g, starts[p], n_subs[p], n_bdys[p], rowblocks[p], ld_rb, subs[p],
                           ld_sub[p], s));
@@ -242,7 +247,10 @@ int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
 
 // Groups of `group_parts` parts, alternating between ctx's stream and its
 // peer context's: one group's SSSP (HBM-bound) runs beside the other's
-// closure and products (VALU-bound).  Synchronous.
+// closure and products (VALU-bound).  Each group's SSSP waits for the
+// previous group's SSSP: two SSSP launches side by side took longer than
+// one after the other (2,048 concurrent sources thrash the caches).
+// Synchronous.
 int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *starts,
                         const int32_t *n_subs, const int32_t *n_bdys, int32_t *const *rowblocks,
                         int64_t ld_rb, int32_t group_parts) {
@@ -271,7 +279,8 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
     fap_ctx *cc = c[gi & 1];
     const int32_t n = std::min(group_parts, count - first);
     rc = enqueue_subgraph_batch(cc, g, n, starts + first, n_subs + first, n_bdys + first,
-                                rowblocks + first, ld_rb, cc->stream);
+                                rowblocks + first, ld_rb, cc->stream,
+                                gi > 0 ? c[(gi - 1) & 1]->ev_sssp : nullptr, cc->ev_sssp);
   }
   const hipError_t e1 = hipStreamSynchronize(ctx->stream);
   const hipError_t e2 = hipStreamSynchronize(peer->stream);
