@@ -715,10 +715,6 @@ __global__ __launch_bounds__(NT) void sssp_group_kernel(
 
 namespace fapi {
 
-// Threshold width: 8 mean arc weights, the best of 1..32 x mean measured on
-// the 512 x 512 road-like grid (w 1..100) and the 64 x 64 unit grid
-// (bench.py --workload ssspsweep).
-
 // Rows go to d_out + i * ld_out, or to d_rowptr[i] (a device array of row
 // pointers) when d_rowptr is not null.
 namespace {

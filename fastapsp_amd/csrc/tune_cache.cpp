@@ -88,11 +88,16 @@ void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, in
   int32_t p[2] = {1024, 8};
   const bool cached = tune_lookup(ctx, "sssp", fap_tune_key_sssp(g->V, g->E, n_src), p);
   // Without a tuned entry: one 1024-thread workgroup per CU while the
-  // sources fit on the CUs, smaller workgroups (more per CU) beyond.
-  // Measured on the 512x512 grid: 74 sources 15.2 ms at 1024 threads vs
-  // 21.2 ms at 256; 3669 sources 187 ms at 256 threads vs 248 ms at 1024.
+  // sources fit on the CUs, smaller workgroups (more per CU) beyond, and a
+  // band that narrows as more sources run at once (fewer re-relaxations
+  // where the memory system is the limit, fewer rounds where it is not).
+  // Measured on the 512x512 grid: 74 sources 10.2 ms at 1024 threads vs
+  // 18.2 ms at 256, band 8 x mean weight best (10.9 ms at 4 x); 3669
+  // sources 119 ms at 256 threads vs 165 ms at 1024, band 3-4 x best
+  // (110 ms, 120 ms at 8 x).
   const int cus = ctx ? ctx->num_cus : 256;
   const int by_count = n_src <= cus ? 1024 : n_src <= 2 * cus ? 512 : 256;
+  const double mult_by_count = n_src <= cus ? 8.0 : n_src <= 2 * cus ? 6.0 : 4.0;
   *threads = (ctx && ctx->opt_sssp_threads > 0) ? ctx->opt_sssp_threads : cached ? p[0] : by_count;
   // Grouped sources only on request: on the same grid a group of 8 takes
   // ~10x one source (its frontiers already keep a workgroup's lanes busy),
@@ -106,7 +111,7 @@ void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, in
     *delta = 1;
   } else {
     const double mean = (double)g->weight_sum / (double)g->E;
-    const double mult = cached ? (double)p[1] : 8.0;
+    const double mult = cached ? (double)p[1] : mult_by_count;
     *delta = (int)std::max(1.0, std::min(1.0e6, mult * mean));
   }
 }

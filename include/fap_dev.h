@@ -69,7 +69,8 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "fw_tile"      0/64/128 FW block size [64]
  *   "sssp_threads" 0/256/512/1024  threads per SSSP workgroup [1024 while the sources
  *                           fit on the CUs, 512 up to twice that, 256 beyond]
- *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 x mean weight]
+ *   "sssp_delta"   >= 0     near-far band width [graph's own, else 8 / 6 / 4 x mean
+ *                           weight for sources up to the CU count / twice it / more]
  *   "sssp_group"   0/1/4/8  sources per SSSP workgroup pass [1]
  *   "sssp_bin"     0..3     reorder SSSP near queues by vertex block when they hold at
  *                           least 2x / 1x / 0.5x the workgroup's threads (1 / 2 / 3);
@@ -118,8 +119,9 @@ int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g,
                             const int32_t *graph_id, const int32_t *st2ed,
                             const int32_t *ed2st);
 int fap_graph_free(fap_ctx *ctx, fap_graph_dev *g);
-/* Near-far band width for fap_sssp_multi on this graph (0 = 8 x mean arc
- * weight).  Affects speed only, never the distances. */
+/* Near-far band width for fap_sssp_multi on this graph (0 = the context's
+ * choice: option sssp_delta, the tune cache, else 8 / 6 / 4 x mean arc
+ * weight by source count).  Affects speed only, never the distances. */
 int fap_graph_set_delta(fap_graph_dev *g, int32_t delta);
 int fap_graph_info(const fap_graph_dev *g, int32_t *V, int64_t *E);
 
