@@ -58,6 +58,22 @@ def test_serve_graphs_match_oracle():
         assert st["fw_ops"] == sum(2 * n ** 3 for n in sizes)
 
 
+def test_serve_large_graph_dense_path_matches_oracle():
+    """A graph of at least 3,072 vertices is closed by the dense blocked FW
+    rather than the wave's batched closure; submitted with a small graph
+    (normally the same wave) and alone, both match the oracle in full."""
+    big, small = rgg(3100, 41), rgg(200, 42)
+    want_big, want_small = apsp(*big), apsp(*small)
+    with fa.Server(hbm_budget=256 << 20, lanes=1) as srv:
+        ids = [srv.submit_graph(*big), srv.submit_graph(*small), srv.submit_graph(*big)]
+        for rid, want in zip(ids, (want_big, want_small, want_big)):
+            info = srv.wait(rid)
+            assert info["state"] == "done"
+            np.testing.assert_array_equal(srv.rows(rid), want)
+            srv.release(rid)
+        assert srv.stats()["failed"] == 0
+
+
 def test_serve_queries_match_oracle():
     ro, col, w = grid(40, 5)
     V = len(ro) - 1
