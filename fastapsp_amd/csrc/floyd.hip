@@ -21,29 +21,8 @@ namespace fapk {
 constexpr int FW_DIAG_THREADS = 1024;
 
 // In-LDS helpers for the diagonal tile (1024 threads).  The tile lives in
-// D[T][T + 4] (rows 16-byte aligned).  A thread owns a 1 x 4 strip of a
-// 64 x 64 sub-block: row t >> 4, columns (t & 15) * 4 .. +3.
-
-// Closure of the 64 x 64 sub-block at (o, o), pivots o .. o + nv - 1.  Row k
-// and column k do not change during step k (the diagonal is >= 0), so every
-// thread can update its strip in place; one barrier per pivot.
-template <int LD>
-__device__ __forceinline__ void lds_fw64(int32_t *D, int o, int nv) {
-  const int t = threadIdx.x;
-  const int i = o + (t >> 4), j = o + ((t & 15) << 2);
-  int4 *cp = reinterpret_cast<int4 *>(D + i * LD + j);
-  for (int k = o; k < o + nv; k++) {
-    const int a = D[i * LD + k];
-    const int4 b = *reinterpret_cast<const int4 *>(D + k * LD + j);
-    int4 c = *cp;
-    c.x = imin(c.x, a + b.x);
-    c.y = imin(c.y, a + b.y);
-    c.z = imin(c.z, a + b.z);
-    c.w = imin(c.w, a + b.w);
-    *cp = c;
-    __syncthreads();
-  }
-}
+// D[T][T + 4] (rows 16-byte aligned); lds_fw64 (minplus_tile.h) closes a
+// 64 x 64 sub-block of it.
 
 // C = min(C, A (x) B) for 64 x 64 sub-blocks given by their corners
 // (row, col) in D, K = 64.  All reads happen before the barrier that

@@ -229,6 +229,30 @@ using MpSmall = MpTile<4, 4>;  // 64 x 64
 // strips at rows (t >> 4) + 16q, q < 4, columns (t & 15) * 4.  Row k and
 // column k do not change during pivot k (the diagonal is >= 0), so every
 // thread updates in place; one barrier per pivot.
+// Closure of a 64 x 64 sub-block with 1024 threads: a thread owns the 1 x 4
+// strip at row t >> 4, columns (t & 15) * 4 .. +3 of the sub-block.
+// The sub-block is at (o, o) of D (row stride LD), pivots o .. o + nv - 1.  Row k
+// and column k do not change during step k (the diagonal is >= 0), so every
+// thread can update its strip in place; one barrier per pivot.
+template <int LD>
+__device__ __forceinline__ void lds_fw64(int32_t *D, int o, int nv) {
+  const int t = threadIdx.x;
+  const int i = o + (t >> 4), j = o + ((t & 15) << 2);
+  int4 *cp = reinterpret_cast<int4 *>(D + i * LD + j);
+  for (int k = o; k < o + nv; k++) {
+    const int a = D[i * LD + k];
+    const int4 b = *reinterpret_cast<const int4 *>(D + k * LD + j);
+    int4 c = *cp;
+    c.x = imin(c.x, a + b.x);
+    c.y = imin(c.y, a + b.y);
+    c.z = imin(c.z, a + b.z);
+    c.w = imin(c.w, a + b.w);
+    *cp = c;
+    __syncthreads();
+  }
+}
+
+
 constexpr int BLK_LD = 64 + 4;
 constexpr int BLK_LDS_WORDS = 64 * BLK_LD;
 
