@@ -6,8 +6,14 @@
 // is closed together:
 //
 //  * each matrix (job) is cut into 64 x 64 tiles and its blocked
-//    Floyd-Warshall becomes a sequence of phases -- diag(r), panels(r),
-//    rest(r) for r = 0 .. nb-1 -- each phase a set of independent tasks;
+//    Floyd-Warshall becomes a sequence of phases, each a set of independent
+//    tile tasks.  Pivot rounds go in pairs: diag(r), panels(r), cross(r)
+//    (row and column r+1 with pivot r), diag(r+1), panels(r+1), rest2(r)
+//    (every tile off rows/columns r, r+1, both pivots in one 128-deep
+//    product: C is read and written once per two rounds), cross(r) (row and
+//    column r with pivot r+1, after rest2 has read them); an unpaired last
+//    round is diag, panels, rest.  Min-plus is exact, so the order of the
+//    two pivots' updates changes no bit;
 //  * phase p of job j goes into launch p + off(j) (off = 0 by default;
 //    staggered offsets let some jobs' diagonal phases share a launch with
 //    others' remaining-tile phases, measured slower): a queue of tasks, largest
@@ -39,6 +45,11 @@ namespace fapk {
 constexpr int BATCH_LDS_WORDS =
     (MpSmall::LDS_WORDS > BLK_LDS_WORDS) ? MpSmall::LDS_WORDS : BLK_LDS_WORDS;
 
+// Kernel instance that runs a task kind (-1: any kind).
+__host__ __device__ constexpr int kind_instance(int kind) {
+  return kind == TASK_CROSS_A || kind == TASK_CROSS_B ? TASK_REST : kind;
+}
+
 // Launches of diagonal tasks only: one 1024-thread workgroup per task, a
 // thread per 1 x 4 strip (lds_fw64, as the dense diagonal kernel), about
 // twice as fast per block as the 256-thread closure.
@@ -56,7 +67,7 @@ __global__ __launch_bounds__(1024) void fw_diag_batch_kernel(const BatchJob *__r
     }
     const BatchSeg sg = segs[lo];
     const BatchJob job = jobs[sg.job];
-    const int base = (sg.kind_r >> 2) * BT;
+    const int base = (sg.kind_r >> 3) * BT;
     const int kb = imin(BT, job.n - base);
     int32_t *blk = job.mat + (int64_t)base * job.ld + base;
     for (int idx = t; idx < 64 * 64; idx += 1024) {
@@ -73,15 +84,17 @@ __global__ __launch_bounds__(1024) void fw_diag_batch_kernel(const BatchJob *__r
   }
 }
 
-// KIND >= 0: every task of the launch has that kind (the host checks), so
-// the other paths compile out and the kernel needs fewer registers (the
-// remaining-tile instance runs 6 waves per SIMD instead of 4); KIND = -1
-// handles launches that mix kinds (staggered jobs).
+// KIND >= 0: every task of the launch runs on that instance (the host
+// checks; kind_instance maps the cross kinds onto the single-pivot one), so
+// the other paths compile out and the tile instances fit 64 VGPRs: 8 waves
+// per SIMD, which hides the memory waits of these short (64- or 128-deep)
+// products better than larger per-lane state would (measured: resident
+// A panels at 4 waves ran slower although they moved fewer bytes).  KIND =
+// -1 handles launches that mix kinds (staggered jobs).
 template <int KIND>
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND > 0 ? 6 : 4))) void fw_phase_kernel(
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND > 0 ? 8 : 4))) void fw_phase_kernel(
     const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, int nseg, int ntasks) {
-  constexpr int LDS_WORDS = (KIND == TASK_PANEL || KIND == TASK_REST) ? MpSmall::LDS_WORDS
-                                                                       : BATCH_LDS_WORDS;
+  constexpr int LDS_WORDS = KIND > 0 ? MpSmall::LDS_WORDS : BATCH_LDS_WORDS;
   __shared__ __attribute__((aligned(16))) int32_t lds[LDS_WORDS];
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     // the segment holding task ti: segments tile [0, ntasks) in order
@@ -93,8 +106,8 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
     }
     const BatchSeg sg = segs[lo];
     const BatchJob job = jobs[sg.job];
-    const int kind = KIND >= 0 ? KIND : (sg.kind_r & 3);
-    const int r = sg.kind_r >> 2, idx = ti - sg.first;
+    const int kind = (KIND >= 0 && KIND != TASK_REST) ? KIND : (sg.kind_r & 7);
+    const int r = sg.kind_r >> 3, idx = ti - sg.first;
     const int nb = job.nb, n = job.n;
     const int64_t ld = job.ld;
     int32_t *d = job.mat;
@@ -116,14 +129,35 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
         int32_t *c = d + (int64_t)I * BT * ld + base;
         MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
       }
-    } else if (KIND < 0 || KIND == TASK_REST) {
-      const int m = nb - 1;
+    } else if ((KIND < 0 || KIND == TASK_REST) && kind != TASK_REST2) {
+      // one tile: off row/column r with pivot r, or of row/column q with
+      // pivot p (cross kinds)
+      int I, J, p = r;
+      if (kind == TASK_REST) {
+        const int m = nb - 1;
+        I = idx / m;
+        J = idx - I * m;
+        I += (I >= r);
+        J += (J >= r);
+      } else {
+        p = kind == TASK_CROSS_A ? r : r + 1;
+        cross_tile(idx, nb, p, kind == TASK_CROSS_A ? r + 1 : r, &I, &J);
+      }
+      const int pb = p * BT;
+      MpSmall::run(d + (int64_t)I * BT * ld + pb, ld, d + (int64_t)pb * ld + (int64_t)J * BT,
+                   ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
+                   imin(BT, n - J * BT), imin(BT, n - pb), true, flags, lds);
+    } else if (KIND < 0 || KIND == TASK_REST2) {
+      // pivots r and r + 1 at once: blocks r, r + 1 are adjacent, so the two
+      // A tiles of row I form one 64 x 128 panel and the two B tiles of
+      // column J one 128 x 64 panel
+      const int m = nb - 2;
       int I = idx / m, J = idx - (idx / m) * m;
-      I += (I >= r);
-      J += (J >= r);
-      int32_t *c = d + (int64_t)I * BT * ld + (int64_t)J * BT;
+      I = I < r ? I : I + 2;
+      J = J < r ? J : J + 2;
       MpSmall::run(d + (int64_t)I * BT * ld + base, ld, d + (int64_t)base * ld + (int64_t)J * BT,
-                   ld, c, ld, imin(BT, n - I * BT), imin(BT, n - J * BT), kb, true, flags, lds);
+                   ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
+                   imin(BT, n - J * BT), imin(2 * BT, n - base), true, flags, lds);
     }
     __syncthreads();  // the LDS buffer is reused by the next task
   }
@@ -153,29 +187,43 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
   std::vector<int32_t> off(count, 0);
   stagger = std::max(1, stagger);
   int launches = 0;
+  std::vector<std::vector<int32_t>> ph(count);  // kind | (r << 3) per phase
   for (int i = 0; i < count; i++) {
     const int j = order[i];
     const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
     out->jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], nb};
     out->ops += 2 * (int64_t)n[j] * n[j] * n[j];
-    const int phases = nb > 1 ? 3 * nb : (n[j] > 0 ? 1 : 0);
     off[j] = i % stagger;
-    if (phases) launches = std::max(launches, off[j] + phases);
+    if (n[j] <= 0) continue;
+    std::vector<int32_t> &pj = ph[j];
+    auto add = [&](int kind, int r) { pj.push_back(kind | (r << 3)); };
+    if (nb == 1) add(fapk::TASK_DIAG, 0);
+    for (int r = 0; nb > 1 && r < nb;) {
+      add(fapk::TASK_DIAG, r);
+      add(fapk::TASK_PANEL, r);
+      if (r + 1 < nb) {
+        add(fapk::TASK_CROSS_A, r);
+        add(fapk::TASK_DIAG, r + 1);
+        add(fapk::TASK_PANEL, r + 1);
+        add(fapk::TASK_REST2, r);
+        add(fapk::TASK_CROSS_B, r);
+        r += 2;
+      } else {
+        add(fapk::TASK_REST, r);
+        r += 1;
+      }
+    }
+    launches = std::max(launches, off[j] + (int)pj.size());
   }
   for (int L = 0; L < launches; L++) {
     int32_t first = 0;
     for (int j : order) {
-      if (n[j] <= 0) continue;
-      const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
-      const int phases = nb > 1 ? 3 * nb : 1;
       const int p = L - off[j];
-      if (p < 0 || p >= phases) continue;
-      const int r = nb > 1 ? p / 3 : 0;
-      const int kind = nb > 1 ? p % 3 : fapk::TASK_DIAG;
-      const int cnt = kind == fapk::TASK_DIAG    ? 1
-                      : kind == fapk::TASK_PANEL ? 2 * (nb - 1)
-                                                 : (nb - 1) * (nb - 1);
-      out->segs.push_back(fapk::BatchSeg{j, kind | (r << 2), first, cnt});
+      if (p < 0 || p >= (int)ph[j].size()) continue;
+      const int kind_r = ph[j][p];
+      const int cnt = fapk::batch_task_count(kind_r & 7, out->jobs[j].nb);
+      if (cnt <= 0) continue;
+      out->segs.push_back(fapk::BatchSeg{j, kind_r, first, cnt});
       first += cnt;
     }
     out->launch_tasks.push_back(first);
@@ -191,10 +239,10 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
     const int cnt = desc.launch_tasks[p];
     const int nseg = desc.seg_off[p + 1] - desc.seg_off[p];
     if (cnt <= 0 || nseg <= 0) continue;
-    // one task kind in the whole launch -> the specialised kernel
-    int kind = desc.segs[desc.seg_off[p]].kind_r & 3;
+    // one kernel instance for every task of the launch -> that instance
+    int kind = fapk::kind_instance(desc.segs[desc.seg_off[p]].kind_r & 7);
     for (int q = desc.seg_off[p] + 1; q < desc.seg_off[p + 1]; q++)
-      if ((desc.segs[q].kind_r & 3) != kind) kind = -1;
+      if (fapk::kind_instance(desc.segs[q].kind_r & 7) != kind) kind = -1;
     const dim3 grid(std::min(cnt, cap)), block(fapk::MP_THREADS);
     const fapk::BatchSeg *sg = d_segs + desc.seg_off[p];
     if (kind == fapk::TASK_DIAG)
@@ -205,6 +253,9 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
                          nseg, cnt);
     else if (kind == fapk::TASK_REST)
       hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_REST>, grid, block, 0, s, d_jobs, sg,
+                         nseg, cnt);
+    else if (kind == fapk::TASK_REST2)
+      hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_REST2>, grid, block, 0, s, d_jobs, sg,
                          nseg, cnt);
     else
       hipLaunchKernelGGL(fapk::fw_phase_kernel<-1>, grid, block, 0, s, d_jobs, sg, nseg, cnt);

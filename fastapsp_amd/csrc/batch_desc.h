@@ -26,7 +26,7 @@ struct BatchJob {
 };
 
 // One phase of one job: tasks [first, first + count) of its launch are the
-// tiles of that phase.  kind_r = kind | (pivot block << 2).
+// tiles of that phase.  kind_r = kind | (pivot block << 3).
 struct BatchSeg {
   int32_t job;
   int32_t kind_r;
@@ -34,7 +34,46 @@ struct BatchSeg {
   int32_t count;
 };
 
-enum : int { TASK_DIAG = 0, TASK_PANEL = 1, TASK_REST = 2 };
+// Phases of a job's blocked Floyd-Warshall.  Rounds go in pairs (r, r + 1):
+//   DIAG(r), PANEL(r), CROSS_A(r), DIAG(r+1), PANEL(r+1), REST2(r), CROSS_B(r)
+// where CROSS_A updates row and column r+1 with pivot r (what DIAG and PANEL
+// of round r+1 read), REST2 updates every tile off rows/columns r, r+1 with
+// both pivots in one 128-deep pass (C is read and written once for two
+// rounds), and CROSS_B then updates row and column r with pivot r+1 (after
+// REST2 has read them).  A last unpaired round is DIAG, PANEL, REST.  No
+// task reads a tile that another task of the same phase writes.
+enum : int {
+  TASK_DIAG = 0,
+  TASK_PANEL = 1,
+  TASK_REST = 2,     // single pivot r, tiles off row/column r
+  TASK_CROSS_A = 3,  // pivot r: row r+1 (J != r), column r+1 (I != r, r+1)
+  TASK_CROSS_B = 4,  // pivot r+1: row r (J != r+1), column r (I != r, r+1)
+  TASK_REST2 = 5,    // pivots r, r+1: tiles off rows/columns r, r+1
+};
+
+__host__ __device__ inline int batch_task_count(int kind, int nb) {
+  switch (kind) {
+    case TASK_DIAG: return 1;
+    case TASK_PANEL: return 2 * (nb - 1);
+    case TASK_REST: return (nb - 1) * (nb - 1);
+    case TASK_CROSS_A:
+    case TASK_CROSS_B: return 2 * nb - 3;
+    default: return (nb - 2) * (nb - 2);
+  }
+}
+
+// Tile u of a cross phase with pivot p on row/column q (|p - q| = 1): first
+// row q without column p, then column q without rows p and q.
+__host__ __device__ inline void cross_tile(int u, int nb, int p, int q, int *I, int *J) {
+  if (u < nb - 1) {
+    *I = q;
+    *J = u < p ? u : u + 1;
+  } else {
+    const int v = u - (nb - 1), lo = p < q ? p : q;
+    *I = v < lo ? v : v + 2;
+    *J = q;
+  }
+}
 
 // One graph's dense matrix: INF, arcs in CSR order, zero diagonal.
 struct DenseJob {

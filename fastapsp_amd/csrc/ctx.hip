@@ -252,7 +252,7 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
       return fail(FAP_E_ARG, "sssp_arcs must be 0 (auto), 1, 2 or 4");
     ctx->opt_sssp_arcs = value == 0 ? 2 : (int)value;
   } else if (k == "batch_grid") {
-    if (value < 0 || value > 16) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 16");
+    if (value < 0 || value > 64) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 64");
     ctx->opt_batch_grid = (int)value;
   } else {
     return fail(FAP_E_ARG, "unknown option '" + k + "'");

@@ -260,9 +260,9 @@ int fap_tune_batch(fap_ctx *ctx, int32_t count, const int32_t *n, int32_t reps) 
   }
   const int saved = ctx->opt_batch_grid;
   double best = -1;
-  int32_t bp[1] = {4};
+  int32_t bp[1] = {16};
   int rc = FAP_OK;
-  for (int per_cu : {1, 2, 4, 8, 16}) {
+  for (int per_cu : {4, 8, 16, 32, 64}) {
     ctx->opt_batch_grid = per_cu;
     fap_fw_plan *plan = nullptr;
     rc = fw_plan_create(ctx, count, mats.data(), n, ld.data(), 0, &plan);

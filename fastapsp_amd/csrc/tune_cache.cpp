@@ -35,7 +35,7 @@ int nparams_of(const std::string &kind) {
 bool valid(const std::string &kind, const int32_t *p) {
   if (kind == "fw") return (p[0] == 64 || p[0] == 128) && (p[1] == 0 || p[1] == 1);
   if (kind == "sssp") return (p[0] == 256 || p[0] == 512 || p[0] == 1024) && p[1] >= 1 && p[1] <= 1024;
-  if (kind == "batch") return p[0] >= 1 && p[0] <= 16;
+  if (kind == "batch") return p[0] >= 1 && p[0] <= 64;
   return false;
 }
 
@@ -118,9 +118,11 @@ void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, in
 
 int batch_grid_per_cu(const fap_ctx *ctx, int64_t batch_key) {
   if (ctx && ctx->opt_batch_grid > 0) return ctx->opt_batch_grid;
-  int32_t p[1] = {4};
+  // 16: twice the resident workgroups of the phase kernels (8 per CU), so
+  // the hardware hands out the second half as the first finish
+  int32_t p[1] = {16};
   if (tune_lookup(ctx, "batch", batch_key, p)) return p[0];
-  return 4;
+  return 16;
 }
 
 }  // namespace fapi

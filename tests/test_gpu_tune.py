@@ -34,7 +34,7 @@ def test_tuners_record_entries(tmp_path):
         sizes = [70, 130, 200, 64, 1]
         ctx.tune_batch(sizes, reps=1)
         p, _ = cache.get("batch", TuneCache.key_batch(sizes))
-        assert p[0] in (1, 2, 4, 8, 16)
+        assert p[0] in (4, 8, 16, 32, 64)
         # the options were restored: still auto
         assert ctx.get_option("fw_tile") == 0 and ctx.get_option("fw_lookahead") == -1
         assert ctx.get_option("sssp_threads") == 0 and ctx.get_option("batch_grid") == 0
@@ -74,7 +74,7 @@ def test_sssp_follows_cache_and_stays_exact(params):
         dg.close()
 
 
-@pytest.mark.parametrize("per_cu", [1, 16])
+@pytest.mark.parametrize("per_cu", [1, 16, 64])
 def test_batch_follows_cache_and_stays_exact(per_cu):
     sizes = [70, 130, 200, 64, 1, 129]
     mats = [dense(n, 40 + i) for i, n in enumerate(sizes)]

@@ -36,7 +36,7 @@ def test_set_get_and_value_checks():
     c.set("fw", 13, [64, 0], 20.0)              # replaces
     assert c.get("fw", 13)[0] == (64, 0) and len(c) == 3
     for kind, p in (("fw", [96, 0]), ("fw", [64, 2]), ("sssp", [300, 8]), ("sssp", [256, 0]),
-                    ("batch", [0]), ("batch", [17]), ("fw", [64]), ("nope", [1])):
+                    ("batch", [0]), ("batch", [65]), ("fw", [64]), ("nope", [1])):
         with pytest.raises(fa.FapError):
             c.set(kind, 1, p, 1.0)
     with pytest.raises(fa.FapError):
