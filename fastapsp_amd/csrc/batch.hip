@@ -14,9 +14,7 @@
 //    column r with pivot r+1, after rest2 has read them); an unpaired last
 //    round is diag, panels, rest.  Min-plus is exact, so the order of the
 //    two pivots' updates changes no bit;
-//  * phase p of job j goes into launch p + off(j) (off = 0 by default;
-//    staggered offsets let some jobs' diagonal phases share a launch with
-//    others' remaining-tile phases, measured slower): a queue of tasks, largest
+//  * phase p of every job goes into launch p: a queue of tasks, largest
 //    jobs first, consumed by a persistent grid (workgroup w takes tasks w,
 //    w + grid, ...).  The queue is described by one segment per job phase
 //    (its first task and count), found by binary search, so descriptors
@@ -41,11 +39,7 @@
 
 namespace fapk {
 
-
-constexpr int BATCH_LDS_WORDS =
-    (MpSmall::LDS_WORDS > BLK_LDS_WORDS) ? MpSmall::LDS_WORDS : BLK_LDS_WORDS;
-
-// Kernel instance that runs a task kind (-1: any kind).
+// Kernel instance that runs a task kind.
 __host__ __device__ constexpr int kind_instance(int kind) {
   return kind == TASK_CROSS_A || kind == TASK_CROSS_B ? TASK_REST : kind;
 }
@@ -84,18 +78,19 @@ __global__ __launch_bounds__(1024) void fw_diag_batch_kernel(const BatchJob *__r
   }
 }
 
-// KIND >= 0: every task of the launch runs on that instance (the host
-// checks; kind_instance maps the cross kinds onto the single-pivot one), so
-// the other paths compile out and the tile instances fit 64 VGPRs: 8 waves
-// per SIMD, which hides the memory waits of these short (64- or 128-deep)
-// products better than larger per-lane state would (measured: resident
-// A panels at 4 waves ran slower although they moved fewer bytes).  KIND =
-// -1 handles launches that mix kinds (staggered jobs).
+// Panel, single-pivot (rest and cross kinds) and pair-pass tasks; every
+// task of a launch runs on one instance (KIND = kind_instance of its kind:
+// the phase patterns keep launches homogeneous and the host checks), so
+// each instance fits 64 VGPRs: 8 waves per SIMD, which hides the memory
+// waits of these short (64- or 128-deep) products better than larger
+// per-lane state would (measured: resident A panels at 4 waves, and four
+// 8 x 8-lane tiles sharing A at 4 waves, were no faster although they read
+// fewer bytes).
 template <int KIND>
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND > 0 ? 8 : 4))) void fw_phase_kernel(
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void fw_phase_kernel(
     const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, int nseg, int ntasks) {
-  constexpr int LDS_WORDS = KIND > 0 ? MpSmall::LDS_WORDS : BATCH_LDS_WORDS;
-  __shared__ __attribute__((aligned(16))) int32_t lds[LDS_WORDS];
+  static_assert(KIND == TASK_PANEL || KIND == TASK_REST || KIND == TASK_REST2, "kernel instance");
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     // the segment holding task ti: segments tile [0, ntasks) in order
     int lo = 0, hi = nseg - 1;
@@ -106,7 +101,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
     }
     const BatchSeg sg = segs[lo];
     const BatchJob job = jobs[sg.job];
-    const int kind = (KIND >= 0 && KIND != TASK_REST) ? KIND : (sg.kind_r & 7);
+    const int kind = KIND == TASK_REST ? (sg.kind_r & 7) : KIND;
     const int r = sg.kind_r >> 3, idx = ti - sg.first;
     const int nb = job.nb, n = job.n;
     const int64_t ld = job.ld;
@@ -115,9 +110,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
     const int kb = imin(BT, n - base);
     const int flags =
         ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
-    if ((KIND < 0 || KIND == TASK_DIAG) && kind == TASK_DIAG) {
-      close_block64(d + (int64_t)base * ld + base, ld, kb, lds);
-    } else if ((KIND < 0 || KIND == TASK_PANEL) && kind == TASK_PANEL) {
+    if (KIND == TASK_PANEL) {
       const int32_t *diag = d + (int64_t)base * ld + base;
       if (idx < nb - 1) {
         const int J = idx < r ? idx : idx + 1;
@@ -129,7 +122,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
         int32_t *c = d + (int64_t)I * BT * ld + base;
         MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
       }
-    } else if ((KIND < 0 || KIND == TASK_REST) && kind != TASK_REST2) {
+    } else if (KIND == TASK_REST) {
       // one tile: off row/column r with pivot r, or of row/column q with
       // pivot p (cross kinds)
       int I, J, p = r;
@@ -147,7 +140,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(KIND
       MpSmall::run(d + (int64_t)I * BT * ld + pb, ld, d + (int64_t)pb * ld + (int64_t)J * BT,
                    ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
                    imin(BT, n - J * BT), imin(BT, n - pb), true, flags, lds);
-    } else if (KIND < 0 || KIND == TASK_REST2) {
+    } else {
       // pivots r and r + 1 at once: blocks r, r + 1 are adjacent, so the two
       // A tiles of row I form one 64 x 128 panel and the two B tiles of
       // column J one 128 x 64 panel
@@ -171,7 +164,7 @@ namespace fapi {
 // decreasing size (longest processing time first), each job's phase one
 // segment of consecutive tasks.  A one-block job has only its diagonal.
 void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
-                    FwBatchDesc *out, int stagger) {
+                    FwBatchDesc *out) {
   out->jobs.resize(count);
   out->segs.clear();
   out->seg_off.assign(1, 0);
@@ -180,12 +173,10 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
   std::vector<int32_t> order(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
-  // Job j's phase p goes to launch p + off[j].  Offsets cycle 0..stagger-1
-  // down the size order, so a launch holds the diagonal phase of some large
-  // jobs beside the (much larger) remaining-tile phase of others instead of
-  // a launch of diagonals alone.  Phases of one job stay in order.
-  std::vector<int32_t> off(count, 0);
-  stagger = std::max(1, stagger);
+  // Job j's phase p goes to launch p.  (Offsetting some jobs' phases, so a
+  // launch mixes one job's diagonal with another's remaining tiles,
+  // measured 3% slower.)  Pair and tail patterns align kind for kind
+  // (diag, panels, then cross or rest), so every launch runs one instance.
   int launches = 0;
   std::vector<std::vector<int32_t>> ph(count);  // kind | (r << 3) per phase
   for (int i = 0; i < count; i++) {
@@ -193,7 +184,6 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
     const int nb = (n[j] + fapk::BT - 1) / fapk::BT;
     out->jobs[j] = fapk::BatchJob{mats[j], ld[j], n[j], nb};
     out->ops += 2 * (int64_t)n[j] * n[j] * n[j];
-    off[j] = i % stagger;
     if (n[j] <= 0) continue;
     std::vector<int32_t> &pj = ph[j];
     auto add = [&](int kind, int r) { pj.push_back(kind | (r << 3)); };
@@ -213,14 +203,13 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
         r += 1;
       }
     }
-    launches = std::max(launches, off[j] + (int)pj.size());
+    launches = std::max(launches, (int)pj.size());
   }
   for (int L = 0; L < launches; L++) {
     int32_t first = 0;
     for (int j : order) {
-      const int p = L - off[j];
-      if (p < 0 || p >= (int)ph[j].size()) continue;
-      const int kind_r = ph[j][p];
+      if (L >= (int)ph[j].size()) continue;
+      const int kind_r = ph[j][L];
       const int cnt = fapk::batch_task_count(kind_r & 7, out->jobs[j].nb);
       if (cnt <= 0) continue;
       out->segs.push_back(fapk::BatchSeg{j, kind_r, first, cnt});
@@ -257,8 +246,8 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
     else if (kind == fapk::TASK_REST2)
       hipLaunchKernelGGL(fapk::fw_phase_kernel<fapk::TASK_REST2>, grid, block, 0, s, d_jobs, sg,
                          nseg, cnt);
-    else
-      hipLaunchKernelGGL(fapk::fw_phase_kernel<-1>, grid, block, 0, s, d_jobs, sg, nseg, cnt);
+    else  // build_fw_batch never mixes instances in one launch
+      return fail(FAP_E_STATE, "batched closure: launch " + std::to_string(p) + " mixes task kinds");
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;

@@ -101,11 +101,8 @@ struct FwBatchDesc {
   int64_t key = 0;  // tune cache bucket (fap_tune_key_batch)
 };
 
-// stagger: phase offsets cycle over this many values down the size order
-// (1 = all jobs start together; 3 measured 3% slower on 512 uneven graphs,
-// 25.06 vs 24.35 ms, so 1 is the default)
 void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
-                    FwBatchDesc *out, int stagger = 1);
+                    FwBatchDesc *out);
 // Launch sequence of a batch whose jobs and segments are already on the
 // device (same order as in `desc`).
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
