@@ -260,7 +260,7 @@ int fap_tune_batch(fap_ctx *ctx, int32_t count, const int32_t *n, int32_t reps) 
   }
   const int saved = ctx->opt_batch_grid;
   double best = -1;
-  int32_t bp[1] = {16};
+  int32_t bp[1] = {64};
   int rc = FAP_OK;
   for (int per_cu : {4, 8, 16, 32, 64}) {
     ctx->opt_batch_grid = per_cu;

@@ -118,11 +118,12 @@ void sssp_settings(const fap_ctx *ctx, const fap_graph_dev *g, int32_t n_src, in
 
 int batch_grid_per_cu(const fap_ctx *ctx, int64_t batch_key) {
   if (ctx && ctx->opt_batch_grid > 0) return ctx->opt_batch_grid;
-  // 16: twice the resident workgroups of the phase kernels (8 per CU), so
-  // the hardware hands out the second half as the first finish
-  int32_t p[1] = {16};
+  // 64: 8 times the resident workgroups of the phase kernels (8 per CU) --
+  // for most launches one task per workgroup, so the hardware hands out
+  // tasks as workgroups finish (512 uneven graphs: 18.1 ms vs 19.0 at 16)
+  int32_t p[1] = {64};
   if (tune_lookup(ctx, "batch", batch_key, p)) return p[0];
-  return 16;
+  return 64;
 }
 
 }  // namespace fapi

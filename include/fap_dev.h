@@ -81,7 +81,7 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *                           workspace (2) [1]
  *   "sssp_wgs"     0..8     SSSP workgroups per CU [1024 threads per CU]
  *   "sssp_arcs"    0/1/2/4  arcs each lane relaxes per SSSP step, loads issued together [2]
- *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [16] */
+ *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);

@@ -19,7 +19,7 @@
  * With no entry the defaults are tile 64, no look-ahead, SSSP threads 1024
  * while the sources fit on the CUs (512 up to twice that, 256 beyond),
  * delta 8 x mean weight while the sources fit on the CUs (6 x up to twice
- * that, 4 x beyond), 16 workgroups per CU.
+ * that, 4 x beyond), 64 workgroups per CU.
  *
  * The cache is a text file, one entry per line: `kind key p0 [p1] ms`.
  * A process whose environment names a file in FAP_TUNE_CACHE loads it once
