@@ -140,9 +140,10 @@ int fap_fw_blocked(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t ld,
 
 /* Closure of `count` matrices (d_mats[j]: n[j] x n[j], stride ld[j]; host
  * arrays of device pointers and sizes) together: 64 x 64 tiles, one launch
- * per phase across all matrices (diag / panels / rest of round r), each a
- * persistent grid over a size-ordered task queue.  Same results as
- * fap_fw_blocked on each matrix.  Synchronous. */
+ * per phase across all matrices (pivot rounds in pairs: diag, panels and
+ * cross tiles of each round, one 128-deep pass over the remaining tiles of
+ * both), each a persistent grid over a size-ordered task queue.  Same
+ * results as fap_fw_blocked on each matrix.  Synchronous. */
 int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                    const int64_t *ld, void *stream);
 

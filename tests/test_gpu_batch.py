@@ -1,4 +1,5 @@
-"""Batched closures (one persistent launch, size-ordered task queue) and the
+"""Batched closures (phase-synchronous launches over size-ordered task
+queues, pivot rounds in pairs) and the
 batched dense build, bit-exact against the oracle and against the one-matrix
 path."""
 
