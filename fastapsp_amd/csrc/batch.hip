@@ -222,9 +222,10 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
 }
 
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
-                    const fapk::BatchSeg *d_segs, hipStream_t s) {
+                    const fapk::BatchSeg *d_segs, hipStream_t s, size_t first, size_t last) {
   const int cap = ctx->num_cus * batch_grid_per_cu(ctx, desc.key);
-  for (size_t p = 0; p < desc.launch_tasks.size(); p++) {
+  last = std::min(last, desc.launch_tasks.size());
+  for (size_t p = first; p < last; p++) {
     const int cnt = desc.launch_tasks[p];
     const int nseg = desc.seg_off[p + 1] - desc.seg_off[p];
     if (cnt <= 0 || nseg <= 0) continue;
@@ -255,55 +256,141 @@ int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob 
 
 }  // namespace fapi
 
+// A plan splits its jobs (size order, alternately) into up to two chains of
+// launches on two streams: every job's phases stay in order on its chain,
+// and the chains share no matrix.  The second chain starts half a round
+// pair after the first, so one chain's short, latency-bound launches
+// (diagonal blocks, panels, cross tiles) overlap the other's 128-deep
+// passes instead of running alone.
+constexpr int PLAN_CHAINS = 2;
+constexpr int PLAN_CHAIN_MIN_JOBS = 8;  // fewer jobs: one chain
+constexpr size_t PLAN_CHAIN_OFFSET = 3;  // diag, panels, cross of the first round
+
 struct fap_fw_plan {
   fap_ctx *ctx = nullptr;
-  void *desc = nullptr;  // jobs | segments (device)
-  size_t segs_off = 0;
-  fapi::FwBatchDesc host;
+  int chains = 1;
+  fapi::FwBatchDesc host[PLAN_CHAINS];
+  void *desc[PLAN_CHAINS] = {};  // jobs | segments (device), per chain
+  size_t segs_off[PLAN_CHAINS] = {};
+  hipStream_t side[PLAN_CHAINS] = {};  // chains 1.. run on side[1..]
+  hipEvent_t ev_join[PLAN_CHAINS] = {};
+  hipEvent_t ev_fork = nullptr, ev_offset = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int64_t ops = 0;
 };
 
 namespace fapi {
 
+static int chain_launch(fap_fw_plan *p, int c, hipStream_t s, size_t first = 0,
+                        size_t last = (size_t)-1) {
+  return launch_fw_batch(p->ctx, p->host[c], (const fapk::BatchJob *)p->desc[c],
+                         (const fapk::BatchSeg *)((char *)p->desc[c] + p->segs_off[c]), s, first,
+                         last);
+}
+
 static int plan_launch(fap_fw_plan *p, hipStream_t s) {
-  return launch_fw_batch(p->ctx, p->host, (const fapk::BatchJob *)p->desc,
-                         (const fapk::BatchSeg *)((char *)p->desc + p->segs_off), s);
+  if (p->chains == 1) return chain_launch(p, 0, s);
+  // fork: the side stream starts after what precedes the plan on s, and
+  // after the first chain's first PLAN_CHAIN_OFFSET launches; join: s waits
+  // for the side chain
+  FAP_HIP(hipEventRecord(p->ev_fork, s));
+  for (int c = 1; c < p->chains; c++) FAP_HIP(hipStreamWaitEvent(p->side[c], p->ev_fork, 0));
+  FAP_TRY(chain_launch(p, 0, s, 0, PLAN_CHAIN_OFFSET));
+  FAP_HIP(hipEventRecord(p->ev_offset, s));
+  FAP_TRY(chain_launch(p, 0, s, PLAN_CHAIN_OFFSET));
+  for (int c = 1; c < p->chains; c++) {
+    FAP_HIP(hipStreamWaitEvent(p->side[c], p->ev_offset, 0));
+    FAP_TRY(chain_launch(p, c, p->side[c]));
+    FAP_HIP(hipEventRecord(p->ev_join[c], p->side[c]));
+    FAP_HIP(hipStreamWaitEvent(s, p->ev_join[c], 0));
+  }
+  return FAP_OK;
+}
+
+void fw_plan_destroy(fap_fw_plan *p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->desc[0]) (void)hipDeviceSynchronize();
+  for (int c = 0; c < PLAN_CHAINS; c++) {
+    if (p->desc[c]) (void)hipFree(p->desc[c]);
+    if (p->ev_join[c]) (void)hipEventDestroy(p->ev_join[c]);
+    if (p->side[c]) (void)hipStreamDestroy(p->side[c]);
+  }
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_offset) (void)hipEventDestroy(p->ev_offset);
+  delete p;
+}
+
+static int plan_upload(fap_fw_plan *p, int c) {
+  const FwBatchDesc &h = p->host[c];
+  const size_t jobs_b = sizeof(fapk::BatchJob) * h.jobs.size();
+  p->segs_off[c] = (jobs_b + 255) & ~(size_t)255;
+  const size_t segs_b = sizeof(fapk::BatchSeg) * h.segs.size();
+  const size_t total = p->segs_off[c] + std::max<size_t>(segs_b, 16);
+  std::vector<char> host(total, 0);
+  if (jobs_b) memcpy(host.data(), h.jobs.data(), jobs_b);
+  if (segs_b) memcpy(host.data() + p->segs_off[c], h.segs.data(), segs_b);
+  hipError_t e = hipMalloc(&p->desc[c], total);
+  if (e == hipSuccess) e = hipMemcpy(p->desc[c], host.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "batch plan descriptors", __FILE__, __LINE__);
+  return FAP_OK;
 }
 
 int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int32_t *n,
                    const int64_t *ld, int use_graph, fap_fw_plan **out) {
   fap_fw_plan *p = new fap_fw_plan();
   p->ctx = ctx;
-  build_fw_batch(count, mats, n, ld, &p->host);
-  const size_t jobs_b = sizeof(fapk::BatchJob) * count;
-  p->segs_off = (jobs_b + 255) & ~(size_t)255;
-  const size_t segs_b = sizeof(fapk::BatchSeg) * p->host.segs.size();
-  const size_t total = p->segs_off + std::max<size_t>(segs_b, 16);
-  std::vector<char> host(total, 0);
-  if (jobs_b) memcpy(host.data(), p->host.jobs.data(), jobs_b);
-  if (segs_b) memcpy(host.data() + p->segs_off, p->host.segs.data(), segs_b);
-  hipError_t e = hipMalloc(&p->desc, total);
-  if (e == hipSuccess) e = hipMemcpy(p->desc, host.data(), total, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    if (p->desc) (void)hipFree(p->desc);
-    delete p;
-    return hip_fail(e, "batch plan descriptors", __FILE__, __LINE__);
+  int rc = FAP_OK;
+  hipError_t e = hipSuccess;
+  p->chains = count >= PLAN_CHAIN_MIN_JOBS ? PLAN_CHAINS : 1;
+  if (p->chains == 1) {
+    build_fw_batch(count, mats, n, ld, &p->host[0]);
+  } else {
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+    std::vector<int32_t *> cm[PLAN_CHAINS];
+    std::vector<int32_t> cn[PLAN_CHAINS];
+    std::vector<int64_t> cl[PLAN_CHAINS];
+    for (int i = 0; i < count; i++) {
+      const int c = i % PLAN_CHAINS, j = order[i];
+      cm[c].push_back(mats[j]);
+      cn[c].push_back(n[j]);
+      cl[c].push_back(ld[j]);
+    }
+    for (int c = 0; c < PLAN_CHAINS; c++)
+      build_fw_batch((int32_t)cn[c].size(), cm[c].data(), cn[c].data(), cl[c].data(),
+                     &p->host[c]);
+    // both chains tune and report under the whole batch's bucket
+    for (int c = 0; c < PLAN_CHAINS; c++) p->host[c].key = fap_tune_key_batch(count, n);
+    e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_offset, hipEventDisableTiming);
+    for (int c = 1; c < p->chains && e == hipSuccess; c++) {
+      e = hipStreamCreateWithFlags(&p->side[c], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join[c], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "batch plan streams", __FILE__, __LINE__);
   }
-  if (use_graph && !p->host.segs.empty()) {
+  for (int c = 0; c < p->chains && rc == FAP_OK; c++) {
+    p->ops += p->host[c].ops;
+    rc = plan_upload(p, c);
+  }
+  bool any = false;
+  for (int c = 0; c < p->chains; c++) any = any || !p->host[c].segs.empty();
+  if (rc == FAP_OK && use_graph && any) {
     hipStream_t cs = ctx->aux_stream;  // capture on a stream with no other work queued
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    int rc = FAP_OK;
     if (e == hipSuccess) rc = plan_launch(p, cs);
     hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess && rc == FAP_OK) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess || rc != FAP_OK) {
-      if (p->graph) (void)hipGraphDestroy(p->graph);
-      (void)hipFree(p->desc);
-      delete p;
-      return rc != FAP_OK ? rc : hip_fail(e, "batch plan graph capture", __FILE__, __LINE__);
-    }
+    if (rc == FAP_OK && e != hipSuccess) rc = hip_fail(e, "batch plan graph capture", __FILE__, __LINE__);
+  }
+  if (rc != FAP_OK) {
+    fw_plan_destroy(p);
+    return rc;
   }
   *out = p;
   return FAP_OK;
@@ -315,17 +402,6 @@ int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
     return FAP_OK;
   }
   return plan_launch(p, s);
-}
-
-void fw_plan_destroy(fap_fw_plan *p) {
-  if (!p) return;
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
-  if (p->desc) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(p->desc);
-  }
-  delete p;
 }
 
 static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, const int64_t *ld) {
@@ -380,10 +456,14 @@ int fap_fw_plan_run(fap_fw_plan *plan, void *stream) {
 int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks, int64_t *ops) {
   if (!plan) return fail(FAP_E_ARG, "null plan");
   int64_t t = 0;
-  for (int32_t c : plan->host.launch_tasks) t += c;
-  if (launches) *launches = (int32_t)plan->host.launch_tasks.size();
+  int32_t l = 0;
+  for (int c = 0; c < plan->chains; c++) {
+    for (int32_t x : plan->host[c].launch_tasks) t += x;
+    l += (int32_t)plan->host[c].launch_tasks.size();
+  }
+  if (launches) *launches = l;
   if (tasks) *tasks = t;
-  if (ops) *ops = plan->host.ops;
+  if (ops) *ops = plan->ops;
   return FAP_OK;
 }
 

@@ -106,7 +106,8 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
 // Launch sequence of a batch whose jobs and segments are already on the
 // device (same order as in `desc`).
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
-                    const fapk::BatchSeg *d_segs, hipStream_t s);
+                    const fapk::BatchSeg *d_segs, hipStream_t s,
+                    size_t first = 0, size_t last = (size_t)-1);
 
 // Multi-source SSSP (sssp.hip): source i's distances go to d_out + i * ld_out,
 // or to d_rowptr[i] when d_rowptr is given (a device array of row pointers).

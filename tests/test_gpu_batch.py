@@ -67,12 +67,28 @@ def test_dense_from_graphs_and_batched_apsp(gpu_ctx):
         g.close()
 
 
-def test_fw_plan_graph_replays_match_eager(gpu_ctx):
-    """The captured phase sequence, replayed several times with eager work
-    and synchronisation in between, gives the eager result every time."""
+def _plan_launches(sizes):
+    """Launches of a plan: 8+ jobs split (size order, alternately) into two
+    chains; a chain has as many launches as its longest job has phases
+    (pivot rounds in pairs of 7 phases -- diag, panels, cross, diag,
+    panels, 128-deep rest, cross -- and 3 for an odd last round)."""
+    def phases(n):
+        nb = (n + 63) // 64
+        return 1 if nb == 1 else 7 * (nb // 2) + 3 * (nb % 2)
+    order = sorted(sizes, reverse=True)
+    chains = 2 if len(sizes) >= 8 else 1
+    return sum(max(phases(n) for n in order[c::chains]) for c in range(chains))
+
+
+@pytest.mark.parametrize("sizes", [[300, 64, 1, 129, 700, 65],
+                                   [300, 64, 1, 129, 700, 65, 200, 513, 90, 1000]],
+                         ids=["one-chain", "two-chains"])
+def test_fw_plan_graph_replays_match_eager(gpu_ctx, sizes):
+    """The captured phase sequence (one stream, or two chains of jobs on two
+    streams joined in the graph), replayed several times with eager work and
+    synchronisation in between, gives the eager result every time."""
     import fastapsp_amd as fa
     lib = fa.load()
-    sizes = [300, 64, 1, 129, 700, 65]
     srcs, work = [], []
     for k, n in enumerate(sizes):
         m = synth.dense_matrix(n, min(1.0, 5.0 / n), 1, 1000, 700 + k)
@@ -91,10 +107,7 @@ def test_fw_plan_graph_replays_match_eager(gpu_ctx):
         np.testing.assert_array_equal(e, O.floyd(m))
     plan = gpu_ctx.fw_plan(ptrs, sizes, lds, use_graph=True)
     info = plan.info()
-    # rounds go in pairs of 7 phases (diag, panels, cross, diag, panels,
-    # 128-deep rest, cross); an odd last round has 3
-    nb = (700 + 63) // 64
-    assert info["launches"] == 7 * (nb // 2) + 3 * (nb % 2)
+    assert info["launches"] == _plan_launches(sizes)
     for rep in range(3):
         restore()
         gpu_ctx.sync()
