@@ -149,7 +149,10 @@ int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const in
 
 /* A reusable batch: descriptors uploaded once; with use_graph the phase
  * launches are captured in a hipGraph and each run replays it (the
- * matrices are read and written in place at every run). */
+ * matrices are read and written in place at every run).  A plan of 8 or
+ * more matrices runs them as two chains of launches, the second on a
+ * stream the plan owns, forked from and joined back to the run's stream.
+ * fap_fw_plan_info: launches summed over the chains, tasks, ops. */
 typedef struct fap_fw_plan fap_fw_plan;
 int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                        const int64_t *ld, int use_graph, fap_fw_plan **out);
