@@ -39,11 +39,6 @@
 
 namespace fapk {
 
-// Kernel instance that runs a task kind.
-__host__ __device__ constexpr int kind_instance(int kind) {
-  return kind == TASK_CROSS_A || kind == TASK_CROSS_B ? TASK_REST : kind;
-}
-
 // Launches of diagonal tasks only: one 1024-thread workgroup per task, a
 // thread per 1 x 4 strip (lds_fw64, as the dense diagonal kernel), about
 // twice as fast per block as the 256-thread closure.
@@ -112,30 +107,20 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
         ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
     if (KIND == TASK_PANEL) {
       const int32_t *diag = d + (int64_t)base * ld + base;
-      if (idx < nb - 1) {
-        const int J = idx < r ? idx : idx + 1;
+      int I, J;
+      panel_tile(r, nb, idx, &I, &J);
+      if (I == r) {
         int32_t *c = d + (int64_t)base * ld + (int64_t)J * BT;
         MpSmall::run(diag, ld, c, ld, c, ld, kb, imin(BT, n - J * BT), kb, true, flags, lds);
       } else {
-        const int u = idx - (nb - 1);
-        const int I = u < r ? u : u + 1;
         int32_t *c = d + (int64_t)I * BT * ld + base;
         MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
       }
     } else if (KIND == TASK_REST) {
       // one tile: off row/column r with pivot r, or of row/column q with
       // pivot p (cross kinds)
-      int I, J, p = r;
-      if (kind == TASK_REST) {
-        const int m = nb - 1;
-        I = idx / m;
-        J = idx - I * m;
-        I += (I >= r);
-        J += (J >= r);
-      } else {
-        p = kind == TASK_CROSS_A ? r : r + 1;
-        cross_tile(idx, nb, p, kind == TASK_CROSS_A ? r + 1 : r, &I, &J);
-      }
+      int I, J, p;
+      single_tile(kind, r, nb, idx, &I, &J, &p);
       const int pb = p * BT;
       MpSmall::run(d + (int64_t)I * BT * ld + pb, ld, d + (int64_t)pb * ld + (int64_t)J * BT,
                    ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
@@ -144,10 +129,8 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
       // pivots r and r + 1 at once: blocks r, r + 1 are adjacent, so the two
       // A tiles of row I form one 64 x 128 panel and the two B tiles of
       // column J one 128 x 64 panel
-      const int m = nb - 2;
-      int I = idx / m, J = idx - (idx / m) * m;
-      I = I < r ? I : I + 2;
-      J = J < r ? J : J + 2;
+      int I, J;
+      pair_tile(r, nb, idx, &I, &J);
       MpSmall::run(d + (int64_t)I * BT * ld + base, ld, d + (int64_t)base * ld + (int64_t)J * BT,
                    ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
                    imin(BT, n - J * BT), imin(2 * BT, n - base), true, flags, lds);

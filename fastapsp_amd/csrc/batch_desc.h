@@ -75,6 +75,46 @@ __host__ __device__ inline void cross_tile(int u, int nb, int p, int q, int *I, 
   }
 }
 
+// Task idx of a panel phase: row r without (r, r), then column r.
+__host__ __device__ inline void panel_tile(int r, int nb, int idx, int *I, int *J) {
+  if (idx < nb - 1) {
+    *I = r;
+    *J = idx < r ? idx : idx + 1;
+  } else {
+    const int u = idx - (nb - 1);
+    *I = u < r ? u : u + 1;
+    *J = r;
+  }
+}
+
+// Task idx of a single-pivot phase: tile (I, J) and its pivot block p --
+// rest: every tile off row/column r, pivot r; cross kinds: cross_tile.
+__host__ __device__ inline void single_tile(int kind, int r, int nb, int idx, int *I, int *J,
+                                            int *p) {
+  if (kind == TASK_REST) {
+    const int m = nb - 1, i = idx / m, j = idx - (idx / m) * m;
+    *I = i + (i >= r);
+    *J = j + (j >= r);
+    *p = r;
+  } else {
+    *p = kind == TASK_CROSS_A ? r : r + 1;
+    cross_tile(idx, nb, *p, kind == TASK_CROSS_A ? r + 1 : r, I, J);
+  }
+}
+
+// Task idx of a pair pass: tile (I, J) off rows/columns r and r + 1.
+__host__ __device__ inline void pair_tile(int r, int nb, int idx, int *I, int *J) {
+  const int m = nb - 2, i = idx / m, j = idx - (idx / m) * m;
+  *I = i < r ? i : i + 2;
+  *J = j < r ? j : j + 2;
+}
+
+// Kernel instance that runs a task kind (the cross kinds run on the
+// single-pivot instance); every task of a launch runs on one instance.
+__host__ __device__ constexpr int kind_instance(int kind) {
+  return kind == TASK_CROSS_A || kind == TASK_CROSS_B ? TASK_REST : kind;
+}
+
 // One graph's dense matrix: INF, arcs in CSR order, zero diagonal.
 struct DenseJob {
   const int32_t *row_offset;

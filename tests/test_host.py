@@ -180,3 +180,25 @@ def test_bench_workloads_are_defined():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_batched_closure_schedule_simulated_on_cpu(tmp_path):
+    """The batched closure's plan builder (from the library) and the task ->
+    tile maps its kernels use, run as a CPU simulation over ragged batches
+    (tests/cpp/batch_schedule_check.cpp): tiles written once per launch, no
+    launch reads a tile it writes elsewhere, one kernel instance per launch,
+    and every matrix equal to plain Floyd-Warshall."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("no hipcc")
+    libdir = os.path.join(ROOT, "fastapsp_amd", "_lib")
+    exe = str(tmp_path / "batch_schedule_check")
+    cmd = [hipcc, "-std=c++17", "-O2", os.path.join(ROOT, "tests", "cpp", "batch_schedule_check.cpp"),
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "fastapsp_amd", "csrc"),
+           "-L" + libdir, "-lfap_amd", "-Wl,-rpath," + libdir, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
