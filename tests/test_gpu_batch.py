@@ -143,3 +143,29 @@ def test_fw_batched_paired_rounds_match_single_graph_fw(gpu_ctx):
         assert (got[:, n:] == -7).all()
         w.free()
         p.free()
+
+
+def test_fw_plan_two_chains_with_an_empty_chain(gpu_ctx):
+    """Eight jobs of which seven are empty: the second chain has no launch,
+    and its branch of the captured graph is only the fork and the join;
+    every replay still closes the one matrix exactly."""
+    import fastapsp_amd as fa
+    lib = fa.load()
+    n = 200
+    m = synth.dense_matrix(n, 5.0 / n, 1, 1000, 4242)
+    src = gpu_ctx.to_device(m)
+    work = gpu_ctx.alloc((n, n))
+    sizes = [n] + [0] * 7
+    ptrs = [work.ptr] + [0] * 7
+    plan = gpu_ctx.fw_plan(ptrs, sizes, sizes, use_graph=True)
+    assert plan.info()["launches"] == _plan_launches(sizes)
+    want = O.floyd(m)
+    for rep in range(3):
+        fa.check(lib.fap_copy_i32(gpu_ctx.handle, work.ptr, src.ptr, n * n, None))
+        gpu_ctx.sync()
+        plan.run()
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(work.get().reshape(n, n), want, err_msg=f"replay {rep}")
+    plan.close()
+    src.free()
+    work.free()
