@@ -242,6 +242,7 @@ struct Lane {
   char *h_stage = nullptr;  // pinned host
   char *d_stage = nullptr;  // device
   hipEvent_t done = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;  // big graphs beside the batched closure
   bool busy = false;
   int kind = FAP_REQ_GRAPH;
   std::vector<Request *> wave;
@@ -431,10 +432,23 @@ int launch_wave(fap_server *S, Lane &L) {
     FAP_HIP(hipMemcpyAsync(L.d_stage, h, off, hipMemcpyHostToDevice, s));
     FAP_TRY(launch_dense_jobs(L.ctx, (const fapk::DenseJob *)(L.d_stage + o_dj), cnt, cells, rows,
                               s));
-    for (int32_t j : big) FAP_TRY(launch_fw(L.ctx, mats[j], ns[j], lds[j], s));
+    // With both kinds in the wave, the big graphs run on the lane
+    // context's auxiliary stream beside the batched closure of the small
+    // ones; the wave's completion (recorded on s) waits for both.
+    const bool split = !big.empty() && !bmats.empty();
+    hipStream_t sb = split ? L.ctx->aux_stream : s;
+    if (split) {
+      FAP_HIP(hipEventRecord(L.fork, s));
+      FAP_HIP(hipStreamWaitEvent(sb, L.fork, 0));
+    }
+    for (int32_t j : big) FAP_TRY(launch_fw(L.ctx, mats[j], ns[j], lds[j], sb));
     if (!bmats.empty())
       FAP_TRY(launch_fw_batch(L.ctx, fd, (const fapk::BatchJob *)(L.d_stage + o_bj),
                               (const fapk::BatchSeg *)(L.d_stage + o_sg), s));
+    if (split) {
+      FAP_HIP(hipEventRecord(L.join, sb));
+      FAP_HIP(hipStreamWaitEvent(s, L.join, 0));
+    }
   } else if (S->index.ready) {
     std::vector<int32_t> qs, qd;
     for (Request *r : L.wave)
@@ -726,6 +740,8 @@ void free_server(fap_server *S) {
   for (Lane &L : S->lanes) {
     if (L.ctx) (void)hipStreamSynchronize(L.ctx->stream);
     if (L.done) (void)hipEventDestroy(L.done);
+    if (L.fork) (void)hipEventDestroy(L.fork);
+    if (L.join) (void)hipEventDestroy(L.join);
     if (L.h_stage) (void)hipHostFree(L.h_stage);
     if (L.d_stage) (void)hipFree(L.d_stage);
     if (L.ctx) (void)fap_ctx_destroy(L.ctx);
@@ -784,6 +800,8 @@ int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out) {
     e = hipHostMalloc((void **)&L.h_stage, (size_t)o.stage_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void **)&L.d_stage, (size_t)o.stage_bytes);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.join, hipEventDisableTiming);
   }
   if (e != hipSuccess || rc != FAP_OK) {
     if (e != hipSuccess) {
