@@ -332,7 +332,8 @@ def bench_grid_pipeline(args):
 
 def bench_grid_batch(args):
     """Plain vs batched per-part solve on the 512x512 road-like grid with
-    K=256.  Row blocks for --call-parts parts stay allocated (1 GiB each);
+    K=256.  Row blocks for --call-parts parts stay allocated (~1.1 GiB each;
+    128 parts: ~147 GB of the 288 GB HBM);
     the parts go through in calls of that many: plain = part after part
     (fap_subgraph_solve); batched = groups of --group parts, one group per
     call of fap_subgraph_solve_batch (one SSSP launch for every boundary
@@ -855,7 +856,7 @@ def main():
     ap.add_argument("--parts-k", type=int, default=256)
     ap.add_argument("--max-parts", type=int, default=0)
     ap.add_argument("--group", type=int, default=32)
-    ap.add_argument("--call-parts", type=int, default=96)
+    ap.add_argument("--call-parts", type=int, default=128)
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate-frac", type=float, default=0.5)
     ap.add_argument("--lanes", type=int, default=3)
