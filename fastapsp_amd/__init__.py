@@ -135,12 +135,16 @@ _SIGS = {
     "fapg_get_all_distance_f32": (_int, [_vp, _pf32]),
     "fapg_get_distance_from_one_point": (_int, [_vp, _i32, _pf32]),
     "fapg_distance_p2p": (_int, [_vp, _i32, _i32, _pi32]),
+    "fapg_get_subgraph_rows_i32": (_int, [_vp, _i32, _i32, _i32, _pi32]),
     "fapg_subgraph_checksum": (_int, [_vp, _i32, _pu64, _pu64]),
     "fapg_get_subgraph_path": (_int, [_vp, _i32, _pi32]),
     "fapg_get_all_path": (_int, [_vp, _pi32]),
     "fapg_get_path_from_one_point": (_int, [_vp, _i32, _pi32]),
     "fapg_path_p2p": (_int, [_vp, _i32, _i32, _pi32, _i32, _pi32]),
     "fap_pred_rows": (_int, [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _i64, _vp]),
+    "fap_minplus_path": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                                _i32, _i32, _i32, _vp]),
+    "fap_fw_blocked_path": (_int, [_vp, _vp, _i32, _i64, _vp, _i64, _vp]),
     "fap_subgraph_solve_groups": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, _vp, _i64, _i32]),
     "fap_subgraph_pred": (_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
@@ -413,6 +417,15 @@ class Context:
 
     def fw(self, ptr: int, n: int, ld: int, stream=None):
         check(load().fap_fw_blocked(self.handle, ptr, n, ld, stream), "fap_fw_blocked")
+
+    def minplus_path(self, A: int, lda: int, B: int, ldb: int, Bp: int, ldbp: int, C_: int,
+                     ldc: int, Cp: int, ldcp: int, M: int, N: int, K: int, stream=None):
+        check(load().fap_minplus_path(self.handle, A, lda, B, ldb, Bp, ldbp, C_, ldc, Cp, ldcp,
+                                      M, N, K, stream), "fap_minplus_path")
+
+    def fw_path(self, ptr: int, n: int, ld: int, path: int, ldp: int, stream=None):
+        check(load().fap_fw_blocked_path(self.handle, ptr, n, ld, path, ldp, stream),
+              "fap_fw_blocked_path")
 
     def fw_batched(self, ptrs, ns, lds, stream=None):
         k = len(ptrs)

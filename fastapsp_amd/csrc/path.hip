@@ -14,9 +14,10 @@
 // cell, the smallest such u with w > 0 (so d[i][u] < d[i][j]; following
 // predecessors strictly decreases the distance and ends at the row's vertex).
 // Cells whose only tight arcs have weight 0 (a plateau of equal distance) are
-// marked and resolved in further passes: a marked cell takes a tight
-// zero-weight in-neighbour that is already resolved, so walks through a
-// plateau end at a vertex resolved earlier.  Which valid predecessor is
+// marked and resolved in further passes: a marked cell takes the smallest
+// tight zero-weight in-neighbour resolved in an earlier pass, so walks
+// through a plateau end at a vertex resolved earlier, and the rows are a
+// function of the distances alone (the same on every run).  Which valid predecessor is
 // chosen when several exist is not fixed by the reference (its CPU, CUDA and
 // HIP paths differ); validity is.
 //
@@ -74,7 +75,10 @@ __global__ void pred_kernel(const int32_t *__restrict__ dist, int64_t ld,
 }
 
 // One resolution pass over plateau cells: take the smallest tight
-// zero-weight in-neighbour already resolved.  Counts the cells resolved.
+// zero-weight in-neighbour resolved BEFORE this pass.  A cell resolved here
+// is written as -(u + 3) (readers of the same pass skip it) and made
+// positive by pred_settle_kernel after the pass, so the choice does not
+// depend on the order in which workgroups run.  Counts the cells resolved.
 __global__ void pred_plateau_kernel(const int32_t *__restrict__ dist, int64_t ld, int32_t nrows,
                                     int32_t V, const int32_t *__restrict__ in_off,
                                     const int2 *__restrict__ in_arcs, int32_t *pred, int64_t ldp,
@@ -91,16 +95,27 @@ __global__ void pred_plateau_kernel(const int32_t *__restrict__ dist, int64_t ld
     int p = 0x7FFFFFFF;
     for (int e = e0; e < e1; e++) {
       const int2 a = in_arcs[e];
-      if (a.y == 0 && d[a.x] == dj && __hip_atomic_load(pr + a.x, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT) != kUnresolved)
+      if (a.y == 0 && d[a.x] == dj &&
+          __hip_atomic_load(pr + a.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= -1)
         p = min(p, a.x);
     }
     if (p != 0x7FFFFFFF) {
-      __hip_atomic_store(pr + j, p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(pr + j, -(p + 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       n++;
     }
   }
   if (n) atomicAdd(resolved, n);
+}
+
+// After a plateau pass: cells resolved in it (-(u + 3)) become u.
+__global__ void pred_settle_kernel(int32_t *pred, int64_t ldp, int32_t nrows, int32_t V) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= V) return;
+  for (int i = blockIdx.y; i < nrows; i += gridDim.y) {
+    int32_t *c = pred + (int64_t)i * ldp + j;
+    const int32_t v = *c;
+    if (v <= -3) *c = -v - 3;
+  }
 }
 
 }  // namespace fapk
@@ -178,6 +193,8 @@ int run_pred_rows(fap_ctx *ctx, fap_graph_dev *g, const int32_t *d_dist, int64_t
     FAP_TRY(launch_fill_u32(cnt, 1, 0u, s));
     hipLaunchKernelGGL(fapk::pred_plateau_kernel, grd, blk, 0, s, d_dist, ld, nrows, V,
                        g->in_offset, g->in_arcs, d_pred, ldp, cnt);
+    FAP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(fapk::pred_settle_kernel, grd, blk, 0, s, d_pred, ldp, nrows, V);
     FAP_CHECK_LAUNCH();
     FAP_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
     FAP_HIP(hipStreamSynchronize(s));

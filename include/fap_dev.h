@@ -19,6 +19,8 @@
  *                          floyd_AMD_path           fap/kernel/hip/hip_kernel.h:24
  *   fap_minplus         <- min_plus_path_advanced   fap/kernel/minplus.h:78-89
  *                          minplus_AMD_path         fap/kernel/hip/hip_kernel.h:40-41
+ *   fap_minplus_path    <- min_plus_path_CPU_advanced (with path) fap/kernel/minplus.h:51-65
+ *   fap_fw_blocked_path <- floyd_CPU_path (with path)  fap/kernel/floyd.h:38-49
  *   fap_gather_submat   <- MysubMatBuild_path       fap/utils/util_Myalgorithm.h:30-73
  *   (A operand view)    <- MyMat1Build              fap/utils/util_Myalgorithm.h:88-95
  *   fap_scatter_submat  <- MysubMatDecode_path      fap/utils/util_Myalgorithm.h:75-86
@@ -231,6 +233,26 @@ int fap_pred_rows(fap_ctx *ctx, fap_graph_dev *g, const int32_t *d_dist, int64_t
 int fap_subgraph_pred(fap_ctx *ctx, fap_graph_dev *g, int32_t start, int32_t n_sub,
                       const int32_t *rowblock, int64_t ld_rb, int32_t *predblock, int64_t ld_pred,
                       void *stream);
+
+/* Min-plus product with the path matrix of min_plus_path_CPU_advanced
+ * (fap/kernel/minplus.h:51-65): C = min(INF, A (x) B) and, when C_path is
+ * not NULL, C_path[i][j] = B_path[k][j] for the first k attaining the
+ * minimum (the reference's strict relaxation in k order), -1 where C is
+ * INF -- bit-identical to the reference for any inputs. */
+int fap_minplus_path(fap_ctx *ctx, const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb,
+                     const int32_t *B_path, int64_t ldbp, int32_t *C, int64_t ldc,
+                     int32_t *C_path, int64_t ldcp, int32_t M, int32_t N, int32_t K,
+                     void *stream);
+/* Floyd-Warshall closure with the path matrix of floyd_CPU_path
+ * (fap/kernel/floyd.h:38-49, path[i][j] = path[k][j] on each strict
+ * improvement): d_path holds the caller's labels on entry.  A cell the
+ * closure does not improve keeps its label; an improved cell takes the
+ * input label of m -> j for a last hop of a shortest i -> j path in the
+ * input matrix, chosen so that the hops lead back to i (which hop among
+ * equal paths is not fixed: the reference's backends differ there).  Uses
+ * 12 n^2 bytes of context workspace; synchronous. */
+int fap_fw_blocked_path(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t ld, int32_t *d_path,
+                        int64_t ldp, void *stream);
 
 #ifdef __cplusplus
 }

@@ -14,8 +14,16 @@
  *   fap_floyd_path               <- fap::floyd_path           fap/kernel/floyd.h:150-158
  *   fap_min_plus_path_advanced   <- fap::min_plus_path_advanced fap/kernel/minplus.h:78-89
  *
- * Path arguments: may be NULL; path matrices are not produced yet and are
- * left untouched (as the reference's HIP SSSP leaves subGraph_path).
+ * Path arguments: may be NULL (no path computed).  When given they are
+ * written with the reference's meaning: subGraph_path rows = Dijkstra
+ * parents (fap/kernel/batch_sssp.h:82-83,110: the source itself at the
+ * source, -1 where unreachable); floyd_path's path holds the caller's labels
+ * on entry and each improved cell takes path[m][j] of a last hop m -> j of a
+ * shortest path (fap/kernel/floyd.h:44); min_plus_path_advanced's path is
+ * mat2_path[k][j] of the first minimising k, -1 where res is INF
+ * (fap/kernel/minplus.h:57-60), exactly as the reference's CPU code.  Which
+ * parent or hop is taken among equally short paths is not fixed (the
+ * reference's CPU, CUDA and HIP backends differ there too).
  * Errors: a FAP_E_* code and fap_last_error(); the C++ wrappers in
  * fap/kernel.h print it and exit(EXIT_FAILURE), like the reference's HIP
  * backend (fap/kernel/hip/batch_sssp.cpp:13-22).

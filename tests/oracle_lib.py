@@ -232,6 +232,34 @@ def ref_min_plus(A, B):
     return Cm
 
 
+def ref_min_plus_path(A, B, B_path):
+    """The reference's min_plus_path_CPU_advanced with its path matrix."""
+    A, B = to_ref(A), to_ref(B)
+    M, K = A.shape
+    N = B.shape[1]
+    Cm = np.empty((M, N), np.float32)
+    cp = np.empty((M, N), np.int32)
+    ref().ref_min_plus_cpu_advanced(_f(A), _f(B), _p(i32(B_path)), _f(Cm), _p(cp), _i(M), _i(N),
+                                    _i(K))
+    return Cm, cp
+
+
+def fw_hop_check(d_in, d_out, hop):
+    """Cells of an FW hop matrix (identity labels: hop[i][j] = the m of the
+    last hop m -> j, or the input label where the cell was not improved)
+    that break the closure's path contract; 0 = valid.  Checked as
+    predecessor rows of the input matrix's digraph (arc m -> j of weight
+    d_in[m][j] for every finite off-diagonal entry) by the oracle's checker,
+    which also walks every hop chain back to its row."""
+    d_in = i32(d_in)
+    n = d_in.shape[0]
+    fin = (d_in < INF) & ~np.eye(n, dtype=bool)
+    ro = np.concatenate([[0], np.cumsum(fin.sum(1))]).astype(np.int32)
+    col = np.nonzero(fin)[1].astype(np.int32)
+    w = d_in[fin].astype(np.int32)
+    return check_pred(d_out, hop, np.arange(n, dtype=np.int32), ro, col, w)
+
+
 def ref_precondition(K, graph_id, ro, col):
     V = len(ro) - 1
     ro = i32(ro)

@@ -150,6 +150,72 @@ def test_dijkstra_and_floyd_vs_bellman_ford(seed):
     np.testing.assert_array_equal(O.floyd(dense_of(n, ro, col, w)), bf)
 
 
+def test_min_plus_path_vs_numpy_first_argmin():
+    """min_plus_path_CPU_advanced's label is B_path[k][j] of the FIRST k
+    attaining the minimum (strict relaxation in k order,
+    fap/kernel/minplus.h:51-65), -1 where the result is INF.  Small weights
+    force ties, so a last-k or any-k rule fails."""
+    rng = np.random.default_rng(3)
+    for M, N, K in ((17, 23, 9), (40, 31, 64), (5, 7, 0)):
+        A = rng.integers(0, 4, (M, K)).astype(np.int32)
+        B = rng.integers(0, 4, (K, N)).astype(np.int32)
+        A[rng.random((M, K)) < 0.3] = O.INF
+        B[rng.random((K, N)) < 0.3] = O.INF
+        A[0, :] = O.INF                       # an all-INF row
+        Bp = rng.integers(-5, 1000, (K, N)).astype(np.int32)
+        Cm, Cp = O.min_plus(A, B, Bp)
+        if K:
+            S = A[:, :, None].astype(np.int64) + B[None, :, :]
+            want = np.minimum(S.min(1), O.INF)
+            k = S.argmin(1)
+            wantp = np.where(want < O.INF, Bp[k, np.arange(N)[None, :]], -1)
+        else:
+            want = np.full((M, N), O.INF)
+            wantp = np.full((M, N), -1)
+        np.testing.assert_array_equal(Cm, want)
+        np.testing.assert_array_equal(Cp, wantp)
+        assert (Cp[0] == -1).all()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU build absent")
+def test_oracle_min_plus_path_vs_reference():
+    rng = np.random.default_rng(4)
+    A = rng.integers(0, 5, (33, 20)).astype(np.int32)
+    B = rng.integers(0, 5, (20, 41)).astype(np.int32)
+    A[rng.random(A.shape) < 0.2] = O.INF
+    B[rng.random(B.shape) < 0.2] = O.INF
+    Bp = rng.integers(0, 10000, B.shape).astype(np.int32)
+    Cm, Cp = O.min_plus(A, B, Bp)
+    rc, rp = O.ref_min_plus_path(A, B, Bp)
+    np.testing.assert_array_equal(Cm, O.from_ref(rc))
+    np.testing.assert_array_equal(Cp, rp)
+
+
+def test_fw_hop_check_accepts_floyd_paths_and_flags_mistakes():
+    """The FW hop checker accepts the reference-semantics FW's own hops
+    (orc_floyd with identity labels: path[i][j] = path[k][j]) on matrices
+    with zero-weight entries, and flags planted mistakes."""
+    rng = np.random.default_rng(5)
+    n = 60
+    m = rng.integers(0, 6, (n, n)).astype(np.int32)
+    m[rng.random((n, n)) < 0.85] = O.INF
+    m[:, n - 1] = O.INF                        # vertex n-1 unreachable from the others
+    np.fill_diagonal(m, 0)
+    lab = np.where(m < O.INF, np.arange(n, dtype=np.int32)[:, None], -1).astype(np.int32)
+    np.fill_diagonal(lab, np.arange(n))
+    d, p = O.floyd(m, with_path=lab)
+    assert (m == 0).sum() > n                  # zero-weight arcs present
+    assert O.fw_hop_check(m, d, p) == 0
+    bad = p.copy()
+    i, j = np.argwhere((d < O.INF) & (d < m))[0]
+    bad[i, j] = (bad[i, j] + 1) % n            # a hop that is not tight (or not an arc)
+    assert O.fw_hop_check(m, d, bad) > 0
+    bad = p.copy()
+    i, j = np.argwhere(d >= O.INF)[0]
+    bad[i, j] = i                              # an unreachable cell with a hop
+    assert O.fw_hop_check(m, d, bad) > 0
+
+
 def test_min_plus_acc_vs_numpy():
     rng = np.random.default_rng(5)
     A = rng.integers(0, 50, (9, 11)).astype(np.int32)
