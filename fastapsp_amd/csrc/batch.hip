@@ -258,6 +258,7 @@ struct fap_fw_plan {
   hipStream_t side[PLAN_CHAINS] = {};  // chains 1.. run on side[1..]
   hipEvent_t ev_join[PLAN_CHAINS] = {};
   hipEvent_t ev_fork = nullptr, ev_offset = nullptr;
+  hipEvent_t ev_last = nullptr;  // recorded on the run's stream after each run
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t ops = 0;
@@ -293,9 +294,15 @@ static int plan_launch(fap_fw_plan *p, hipStream_t s) {
 
 void fw_plan_destroy(fap_fw_plan *p) {
   if (!p) return;
+  // the last run may still be in flight: wait for its end on the run's
+  // stream (ev_last, which follows the join of every chain) and for the
+  // plan's side streams before the exec and descriptors go; other streams
+  // of the device are not waited for
+  if (p->ev_last) (void)hipEventSynchronize(p->ev_last);
+  for (int c = 0; c < PLAN_CHAINS; c++)
+    if (p->side[c]) (void)hipStreamSynchronize(p->side[c]);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  if (p->desc[0]) (void)hipDeviceSynchronize();
   for (int c = 0; c < PLAN_CHAINS; c++) {
     if (p->desc[c]) (void)hipFree(p->desc[c]);
     if (p->ev_join[c]) (void)hipEventDestroy(p->ev_join[c]);
@@ -303,6 +310,7 @@ void fw_plan_destroy(fap_fw_plan *p) {
   }
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_offset) (void)hipEventDestroy(p->ev_offset);
+  if (p->ev_last) (void)hipEventDestroy(p->ev_last);
   delete p;
 }
 
@@ -328,6 +336,12 @@ int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int3
   int rc = FAP_OK;
   hipError_t e = hipSuccess;
   p->chains = count >= PLAN_CHAIN_MIN_JOBS ? PLAN_CHAINS : 1;
+  e = hipEventCreateWithFlags(&p->ev_last, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "batch plan event", __FILE__, __LINE__);
+    fw_plan_destroy(p);
+    return rc;
+  }
   if (p->chains == 1) {
     build_fw_batch(count, mats, n, ld, &p->host[0]);
   } else {
@@ -382,9 +396,11 @@ int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int3
 int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
   if (p->exec) {
     FAP_HIP(hipGraphLaunch(p->exec, s));
-    return FAP_OK;
+  } else {
+    FAP_TRY(plan_launch(p, s));
   }
-  return plan_launch(p, s);
+  FAP_HIP(hipEventRecord(p->ev_last, s));
+  return FAP_OK;
 }
 
 static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, const int64_t *ld) {

@@ -570,7 +570,15 @@ void serve_loop(fap_server *S) {
         lk.unlock();
         const int rc = launch_wave(S, L);
         const std::string err = rc == FAP_OK ? std::string() : std::string(fap_last_error());
-        if (rc != FAP_OK) (void)hipStreamSynchronize(L.ctx->stream);
+        if (rc != FAP_OK) {
+          // a failure after the fork of a mixed wave (or inside a closure's
+          // look-ahead) can leave kernels queued on the auxiliary stream
+          // with no join back to the lane's stream: both must be idle before
+          // fail_wave hands the wave's arena blocks back
+          (void)hipStreamSynchronize(L.ctx->stream);
+          (void)hipStreamSynchronize(L.ctx->aux_stream);
+          (void)hipGetLastError();
+        }
         lk.lock();
         if (rc != FAP_OK) fail_wave(S, L, rc, err);
         busy = true;
