@@ -1,32 +1,51 @@
 #!/usr/bin/env python3
 """Benchmark of the subgraph-solve hot path on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on): the
-dense kernel pair on an n = 8192 int32 distance matrix -- each off-diagonal
-entry an arc with probability 0.01 and weight in [1, 1000], else INF,
-diagonal 0 -- i.e. one blocked Floyd-Warshall closure of the matrix in place
-plus one min-plus product of two such matrices.  One step = restore the FW
-input (device copy) + FW + min-plus, inputs resident in HBM.
-value = min-plus GOp/s = (2 n^3 [FW] + 2 n^3 [product]) / step time, 2 ops
-(add, min) per update as the reference counts (fap/kernel/cuda/floyd.cu:340,
-fap/kernel/cuda/minplus.cu:483).
+Default run (what the one JSON line reports):
 
-The roofline object is for the min-plus product kernel, timed with HIP events
-on its own stream over the timed steps; the Floyd-Warshall closure is
-reported beside it.  Min-plus is not a multiply-accumulate, so the bound is
-the VALU, not MFMA.  peak = the measured ceiling of the kernels' instruction
-mix (one v_add_u32 per update, one v_min3_i32 per two), probed on the same
-device in the same run (fap_probe_minplus_ceiling); on gfx950 v_add_u32
-issues at twice the rate of v_min3_i32, so that ceiling (~76 TOP/s) is
-above the spec figure of 1 int32 op per lane per clock, 256 CUs x 64 lanes x
-2.4 GHz = 39.3 T, which is reported beside it as spec_peak.
+* value -- the dense kernel pair on an n = 8192 int32 distance matrix (each
+  off-diagonal entry an arc with probability 0.01 and weight in [1, 1000],
+  else INF, diagonal 0): one blocked Floyd-Warshall closure in place plus
+  one min-plus product of two such matrices.  One step = restore the FW
+  input (device copy) + FW + min-plus, inputs resident in HBM.  value =
+  min-plus GOp/s = (2 n^3 [FW] + 2 n^3 [product]) / step time, 2 ops (add,
+  min) per update as the reference counts (fap/kernel/cuda/floyd.cu:340,
+  fap/kernel/cuda/minplus.cu:483).
+* batch_c4 -- 512 random geometric graphs, n log-uniform in [64, 2048],
+  each closed to full APSP: the plain loop (one graph per launch sequence,
+  the reference's structure, fap/fap.cpp:189-193) against the batched
+  closure replayed from a hipGraph; identical results required.
+* serve_c5 -- a Poisson stream at a FIXED arrival rate (--rate req/s): half
+  whole-graph APSP requests (RGG, n log-uniform in [256, 4096]), half 1-64
+  point queries on the resident 512x512 grid; p50/p99 on the batching
+  server and on a plain one (one lane, one request per wave) at the same
+  rate; identical results required.
+* grid_c3 -- the 512x512 road-like grid, K = 256: one SSSP launch over a
+  group's boundary vertices (edges/s, algorithmic GB/s and its fraction of
+  HBM), and the plain part-by-part solve against the pipelined groups
+  (fapGraph::solve's default) over all 256 parts; identical checksums
+  required.
+
+roofline: the Floyd-Warshall closure (fw_diag/fw_panel/fw_rest kernels of
+one closure; the largest share of the step), timed with HIP events on its
+own stream; roofline_minplus the same for the product kernel.  Min-plus is
+not a multiply-accumulate, so the bound is the VALU, not MFMA.  peak = the
+measured ceiling of the kernels' instruction mix (one v_add_u32 per update,
+one v_min3_i32 per two), probed on the same device in the same run
+(fap_probe_minplus_ceiling); on gfx950 v_add_u32 issues at twice the rate of
+v_min3_i32, so that ceiling (~76 TOP/s) is above the spec figure of 1 int32
+op per lane per clock, 256 CUs x 64 lanes x 2.4 GHz = 39.3 T, which is
+reported beside it as spec_peak.
 
 cpu_baseline: the reference's own CPU code (oracle/_ref, compiled from its
-headers; min_plus_path_CPU_advanced, single-threaded like the reference) on
-a row band of the same product, scaled to GOp/s; the int32 restatement
-(oracle/_build) is used where the reference build is absent.
+headers; floyd_CPU_path and min_plus_path_CPU_advanced, single-threaded like
+the reference) on a bounded sample of the step -- the closure of the
+leading 1024 x 1024 block and a row band of the product -- each scaled to
+the full step by its op count; the int32 restatement (oracle/_build) is used
+where the reference build is absent.  host_cores/host_cpu describe the
+machine it ran on.
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--n 8192]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--dense-only]
 """
 
 from __future__ import annotations
@@ -35,6 +54,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -55,38 +75,72 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_minplus(n: int, A: np.ndarray, B: np.ndarray, rows: int):
-    """GOp/s of the reference CPU min-plus on `rows` rows of the n^3 product."""
+def host_info():
+    """Logical CPUs, physical cores and model of this host (lscpu)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        cps, sk = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sk and cps.isdigit() and sk.isdigit():
+            info["physical_cores"] = int(cps) * int(sk)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline_dense(n: int, M0: np.ndarray, A: np.ndarray, B: np.ndarray, rows: int,
+                       fw_n: int):
+    """GOp/s of the reference CPU path on the dense step, from a sample: the
+    FW closure of M0's leading fw_n x fw_n block (scaled by (n / fw_n)^3) and
+    `rows` rows of the n^3 product (scaled by n / rows)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     a = np.ascontiguousarray(A[:rows])
+    sub = np.ascontiguousarray(M0[:fw_n, :fw_n])
     if O.ref_available():
-        fa = O.to_ref(a)
-        fb = O.to_ref(B)
+        t = time.perf_counter()
+        O.ref_floyd(sub)
+        t_fw = time.perf_counter() - t
+        fa_, fb = O.to_ref(a), O.to_ref(B)
         Cm = np.empty((rows, n), np.float32)
         bp = np.zeros((n, n), np.int32)
         cp = np.empty((rows, n), np.int32)
-        fn = O.ref().ref_min_plus_cpu_advanced
         t = time.perf_counter()
-        fn(O._f(fa), O._f(fb), O._p(bp), O._f(Cm), O._p(cp), C.c_int32(rows), C.c_int32(n),
-           C.c_int32(n))
-        dt = time.perf_counter() - t
+        O.ref().ref_min_plus_cpu_advanced(O._f(fa_), O._f(fb), O._p(bp), O._f(Cm), O._p(cp),
+                                          C.c_int32(rows), C.c_int32(n), C.c_int32(n))
+        t_mp = time.perf_counter() - t
         kind = "reference"
     else:
         t = time.perf_counter()
+        O.floyd(sub)
+        t_fw = time.perf_counter() - t
+        t = time.perf_counter()
         O.min_plus(a, B)
-        dt = time.perf_counter() - t
+        t_mp = time.perf_counter() - t
         kind = "port"
-    ops = 2.0 * rows * n * n
-    return {"value": ops / dt / 1e9, "unit": "GOp/s", "cores": 1, "kind": kind,
-            "sample": f"min_plus_path_CPU_advanced on {rows} of {n} rows of the {n}^3 product "
-                      f"({ops:.3g} ops, {dt:.1f} s), scaled to GOp/s"}
+    full = t_fw * (n / fw_n) ** 3 + t_mp * (n / rows)
+    hi = host_info()
+    return {"value": round(4.0 * n ** 3 / full / 1e9, 4), "unit": "GOp/s", "cores": 1,
+            "kind": kind,
+            "sample": f"floyd_CPU_path on the leading {fw_n}^2 block ({t_fw:.1f} s, x{(n / fw_n) ** 3:.0f}) "
+                      f"+ min_plus_path_CPU_advanced on {rows} of {n} rows of the {n}^3 product "
+                      f"({t_mp:.1f} s, x{n / rows:.0f}); single-threaded like the reference; "
+                      f"scaled step {full:.0f} s",
+            "host_cores": hi.get("physical_cores"), "host_logical_cpus": hi.get("logical_cpus"),
+            "host_cpu": hi.get("model")}
 
 
-def bench_dense(args):
+def dense_section(args, ctx, lib):
+    """The c2 step (FW closure + min-plus product at n), timed with HIP
+    events per kernel; returns the line's main fields."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
-    lib = fa.load()
     n = args.n
     log(f"[bench] generating n={n} matrices")
     t0 = time.time()
@@ -94,7 +148,6 @@ def bench_dense(args):
     A = synth.dense_matrix(n, 0.01, 1, 1000, 21)
     B = synth.dense_matrix(n, 0.01, 1, 1000, 22)
     log(f"[bench] generated in {time.time() - t0:.1f}s")
-    ctx = fa.Context(0)
     ceiling = ctx.probe_minplus_ceiling()
     log(f"[bench] min-plus instruction-mix ceiling {ceiling:.2f} TOP/s")
     tuned = _tune(ctx, args, "fw", fa.TuneCache.key_fw(n), lambda: ctx.tune_fw(n, reps=1))
@@ -133,51 +186,86 @@ def bench_dense(args):
         fa.check(lib.fap_event_elapsed(ev[4 * i + 2], ev[4 * i + 3], C.byref(ms)))
         mp_ms.append(ms.value)
         log(f"[bench] step {i}: fw {fw_ms[-1]:.2f} ms, min-plus {mp_ms[-1]:.2f} ms")
+    for e in ev:
+        lib.fap_event_destroy(e)
+    for d in (d_src, d_fw, dA, dB, dC):
+        d.free()
     ops_step = 4.0 * n ** 3
-    ms_step = 1e3 * total / args.steps
-    value = ops_step / (total / args.steps) / 1e9
     mp_avg = float(np.mean(mp_ms))
     fw_avg = float(np.mean(fw_ms))
     ach = 2.0 * n ** 3 / (mp_avg * 1e-3) / 1e12
     ach_fw = 2.0 * n ** 3 / (fw_avg * 1e-3) / 1e12
-    cpu = None
-    if args.cpu_rows > 0:
-        log("[bench] cpu baseline ...")
-        cpu = cpu_baseline_minplus(n, A, B, rows=args.cpu_rows)
     res = {
         "metric": "min-plus GOp/s",
-        "value": round(value, 1),
+        "value": round(ops_step / (total / args.steps) / 1e9, 1),
         "unit": "GOp/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(1e3 * total / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"dense FW + min-plus product, n={n}, arc p=0.01, w 1..1000",
-                   "n": n, "ops_per_step": ops_step},
-        "roofline": {"bound": "valu", "kernel": "minplus_gemm_kernel",
-                     "achieved": round(ach, 2), "peak": round(ceiling, 2),
-                     "unit": "TOP/s", "frac": round(ach / ceiling, 3),
-                     "traffic": MINPLUS_8192_TRAFFIC if n == 8192 else None,
+        "config": {"workload": f"dense FW closure + min-plus product, n={n}, arc p=0.01, "
+                               f"w 1..1000 (value); batch_c4, serve_c5 and grid_c3 carry the "
+                               f"batched/served-vs-plain forms", "n": n, "ops_per_step": ops_step},
+        "roofline": {"bound": "valu",
+                     "kernel": "Floyd-Warshall closure (fw_rest phase-3 tiles with the fused "
+                               "next diagonal, fw_panel, fw_diag; one closure per step)",
+                     "achieved": round(ach_fw, 2), "peak": round(ceiling, 2), "unit": "TOP/s",
+                     "frac": round(ach_fw / ceiling, 3), "traffic": None,
                      "peak_kind": "measured v_add_u32 + v_min3_i32 mix ceiling",
                      "spec_peak": round(VALU_PEAK_TOPS, 2),
-                     "frac_of_spec_peak": round(ach / VALU_PEAK_TOPS, 3),
-                     "ms_per_launch": round(mp_avg, 3)},
-        "roofline_fw": {"bound": "valu", "kernels": "fw_diag/fw_panel/fw_rest",
-                        "achieved": round(ach_fw, 2), "peak": round(ceiling, 2),
-                        "unit": "TOP/s", "frac": round(ach_fw / ceiling, 3),
-                        "spec_peak": round(VALU_PEAK_TOPS, 2),
-                        "frac_of_spec_peak": round(ach_fw / VALU_PEAK_TOPS, 3),
-                        "ms_per_closure": round(fw_avg, 3)},
-        "cpu_baseline": cpu,
+                     "frac_of_spec_peak": round(ach_fw / VALU_PEAK_TOPS, 3),
+                     "ms_per_launch": round(fw_avg, 3), "ops_per_launch": 2.0 * n ** 3},
+        "roofline_minplus": {"bound": "valu", "kernel": "minplus_gemm_kernel",
+                             "achieved": round(ach, 2), "peak": round(ceiling, 2),
+                             "unit": "TOP/s", "frac": round(ach / ceiling, 3),
+                             "traffic": MINPLUS_8192_TRAFFIC if n == 8192 else None,
+                             "spec_peak": round(VALU_PEAK_TOPS, 2),
+                             "frac_of_spec_peak": round(ach / VALU_PEAK_TOPS, 3),
+                             "ms_per_launch": round(mp_avg, 3), "ops_per_launch": 2.0 * n ** 3},
         "tuned": tuned,
     }
+    res["cpu_baseline"] = None
+    if args.cpu_rows > 0:
+        log("[bench] cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline_dense(n, M0, A, B, rows=args.cpu_rows,
+                                                 fw_n=args.cpu_fw_n)
+    return res, ceiling
+
+
+def bench_dense(args):
+    """The driver's line: the dense step as value, plus the batched and
+    served forms against their plain counterparts (c4, c5, c3)."""
+    import fastapsp_amd as fa
+    lib = fa.load()
+    t_all = time.time()
+    ctx = fa.Context(0)
+    res, ceiling = dense_section(args, ctx, lib)
+    rc = 0
+    if not args.dense_only:
+        t0 = time.time()
+        res["batch_c4"] = batch_section(args, ctx, lib, ceiling)
+        log(f"[bench] batch_c4 in {time.time() - t0:.1f} s: {res['batch_c4']}")
+        rc |= 0 if res["batch_c4"]["identical"] else 1
+        ctx.close()
+        grid = grid_setup(args)
+        t0 = time.time()
+        res["serve_c5"] = serve_section(args, grid)
+        log(f"[bench] serve_c5 in {time.time() - t0:.1f} s: {res['serve_c5']}")
+        rc |= 0 if res["serve_c5"]["identical"] else 1
+        t0 = time.time()
+        res["grid_c3"] = grid_section(args, grid)
+        log(f"[bench] grid_c3 in {time.time() - t0:.1f} s: {res['grid_c3']}")
+        rc |= 0 if res["grid_c3"]["identical"] else 1
+    else:
+        ctx.close()
+    res["bench_wall_s"] = round(time.time() - t_all, 1)
     print(json.dumps(res), flush=True)
-    return 0
+    return rc
 
 
 def _tune(ctx, args, kind, key, run):
@@ -330,47 +418,70 @@ def bench_grid_pipeline(args):
     return 0
 
 
-def bench_grid_batch(args):
-    """Plain vs batched per-part solve on the 512x512 road-like grid with
-    K=256.  Row blocks for --call-parts parts stay allocated (~1.1 GiB each;
-    128 parts: ~147 GB of the 288 GB HBM);
-    the parts go through in calls of that many: plain = part after part
-    (fap_subgraph_solve); batched = groups of --group parts, one group per
-    call of fap_subgraph_solve_batch (one SSSP launch for every boundary
-    vertex of the group, one batched closure, the per-part products);
-    pipelined = fap_subgraph_solve_groups over the whole call (groups
-    alternate between two streams, one group's SSSP beside the previous
-    group's closure and products).  Row blocks are checksummed on the
-    device after each call (outside the timed region) and must agree."""
+def grid_setup(args):
+    """The 512x512 road-like grid (10% of edges removed, 2% of V diagonal
+    shortcuts, w 1..100) and its K-part partition (host preCondition)."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
-    R, K, G = args.grid, args.parts_k, args.group
+    R, K = args.grid, args.parts_k
     V = R * R
+    t0 = time.time()
     u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
     ro, col, wt = synth.to_csr(V, u, v, w, False)
-    E = len(col)
     g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), K)
     g.preCondition()
     vn, vo, bn, _ = g.part_info()
     st2ed = g.getMapping()
     gid = g.getGraphId()
+    g.close()
     ed2st = np.empty(V, np.int32)
     ed2st[st2ed] = np.arange(V, dtype=np.int32)
-    g.close()
+    log(f"[bench] grid {R}x{R} K={K}: V={V} E={len(col)}, partitioned in {time.time() - t0:.1f} s")
+    return dict(R=R, K=K, V=V, E=len(col), ro=ro, col=col, wt=wt, vn=vn, vo=vo, bn=bn,
+                st2ed=st2ed, gid=gid, ed2st=ed2st)
+
+
+def grid_section(args, grid, modes=("plain", "pipelined")):
+    """c3 on the grid: (1) one SSSP launch over the boundary vertices of the
+    first --group parts (edges/s, algorithmic bytes 4(V+1) + 8E + 4 S V per
+    launch, their rate as a fraction of HBM peak); (2) all parts solved
+    plain (fap_subgraph_solve part after part), batched (groups of --group
+    parts, fap_subgraph_solve_batch) and/or pipelined (groups alternating
+    over two streams, fap_subgraph_solve_groups -- fapGraph::solve's
+    default), in calls of --call-parts parts whose ~1.1 GiB row blocks stay
+    allocated; row blocks are checksummed on the device after each call
+    (outside the timed region) and must agree between the forms."""
+    import fastapsp_amd as fa
+    lib = fa.load()
+    V, E, K, G = grid["V"], grid["E"], grid["K"], args.group
+    vn, vo, bn, st2ed = grid["vn"], grid["vo"], grid["bn"], grid["st2ed"]
     ctx = fa.Context(0)
-    dg = ctx.upload_graph(ro, col, wt)
-    dg.set_partition(gid, st2ed, ed2st)
+    dg = ctx.upload_graph(grid["ro"], grid["col"], grid["wt"])
+    dg.set_partition(grid["gid"], st2ed, grid["ed2st"])
     parts = list(range(1, K + 1))[: args.max_parts] if args.max_parts else list(range(1, K + 1))
+    srcs = np.concatenate([st2ed[int(vo[p]): int(vo[p]) + int(bn[p])]
+                           for p in parts[:G]]).astype(np.int32)
+    tuned = _tune(ctx, args, "sssp", fa.TuneCache.key_sssp(V, E, len(srcs)),
+                  lambda: ctx.tune_sssp(dg, srcs, reps=1))
+    dsrc = ctx.to_device(srcs)
+    out = ctx.alloc((len(srcs), V))
+    sssp_ms = _time_ms(lib, ctx, lambda: dg.sssp(dsrc.ptr, len(srcs), out.ptr, V, ctx.stream),
+                       reps=3)
+    out.free()
+    dsrc.free()
+    S = len(srcs)
+    sssp_bytes = 4.0 * (V + 1) + 8.0 * E + 4.0 * S * V
+    sssp = {"sources": S, "parts": min(G, len(parts)), "ms_per_launch": round(sssp_ms, 3),
+            "edges_per_s": round(S * E / (sssp_ms * 1e-3), 1),
+            "algorithmic_bytes": sssp_bytes,
+            "algorithmic_GB_per_s": round(sssp_bytes / (sssp_ms * 1e-3) / 1e9, 2),
+            "frac_of_hbm_peak": round(sssp_bytes / (sssp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "tuned": tuned}
+    log(f"[bench] c3 SSSP: {sssp}")
     P = max(G, min(args.call_parts, len(parts)))
     calls = [parts[i:i + P] for i in range(0, len(parts), P)]
     nmax = int(vn[1:].max())
     blks = [ctx.alloc((nmax, V)) for _ in range(min(P, len(parts)))]
-    log(f"grid-batch: V={V} E={E} K={K} parts={len(parts)} group={G} call={P} "
-        f"blocks={len(blks)}x{nmax}x{V}")
-    # SSSP launch shape for a group's boundary vertices, picked by the tuner
-    srcs = np.concatenate([st2ed[int(vo[p]): int(vo[p]) + int(bn[p])] for p in parts[:G]])
-    tuned = _tune(ctx, args, "sssp", fa.TuneCache.key_sssp(V, E, len(srcs)),
-                  lambda: ctx.tune_sssp(dg, srcs, reps=1))
 
     def args_of(ids):
         return ([int(vo[p]) for p in ids], [int(vn[p]) for p in ids], [int(bn[p]) for p in ids],
@@ -387,7 +498,7 @@ def bench_grid_batch(args):
                     dg.subgraph_solve(int(vo[p]), int(vn[p]), int(bn[p]), blks[i].ptr, V)
             elif mode == "batched":
                 for j in range(0, len(call), G):
-                    st, ns, bs, rb = args_of(call[j:j + G])
+                    st, ns, bs, _ = args_of(call[j:j + G])
                     dg.subgraph_solve_batch(st, ns, bs, [blks[j + i].ptr for i in range(len(st))],
                                             V)
             else:
@@ -398,39 +509,46 @@ def bench_grid_batch(args):
             sums += [ctx.checksum(blks[i].ptr, int(vn[p]) * V) for i, p in enumerate(call)]
         return t, sums
 
-    modes = ("plain", "batched", "pipelined")
     for m in modes:
         run(m)  # warm-up (workspaces, kernels, the second stream's context)
     res = {}
     for mode in modes:
         ts = []
-        for _ in range(max(1, args.steps)):
+        for _ in range(max(1, args.grid_steps)):
             t, sums = run(mode)
             ts.append(t)
         res[mode] = (min(ts), sums)
-        log(f"  {mode}: {1e3 * min(ts):.1f} ms for {len(parts)} parts")
+        log(f"[bench] c3 {mode}: {1e3 * min(ts):.1f} ms for {len(parts)} parts")
+    for b in blks:
+        b.free()
+    dg.close()
+    ctx.close()
     ops = sum(2.0 * int(vn[p]) ** 3 + 2.0 * (int(vn[p]) - int(bn[p])) * int(bn[p]) * V
               for p in parts)
-    tp, sp = res["plain"]
-    tb, sb = res["batched"]
-    tq, sq = res["pipelined"]
-    same = sp == sb == sq
-    out = {
-        "metric": "min-plus GOp/s", "value": round(ops / tq / 1e9, 1), "unit": "GOp/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": 1,
-        "ms_per_step": round(1e3 * tq, 2), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"{R}x{R} grid (-10% edges, +2% diagonals, w 1..100), K={K}, "
-                               f"batched per-part solve, groups of {G} on two streams",
-                   "V": V, "E": E, "K": K, "parts": len(parts), "group": G, "call_parts": P},
-        "plain_ms": round(1e3 * tp, 2), "batched_ms": round(1e3 * tb, 2),
-        "pipelined_ms": round(1e3 * tq, 2),
-        "batched_over_plain": round(tp / tb, 2), "pipelined_over_plain": round(tp / tq, 2),
-        "plain_GOp_s": round(ops / tp / 1e9, 1), "batched_GOp_s": round(ops / tb / 1e9, 1),
-        "identical": bool(same), "tuned": tuned,
-    }
-    print(json.dumps(out), flush=True)
-    return 0 if same else 1
+    sums0 = res[modes[0]][1]
+    out = {"workload": f"{grid['R']}x{grid['R']} grid (-10% edges, +2% diagonals, w 1..100), "
+                       f"K={K}, every part's n_sub x V row block",
+           "V": V, "E": E, "K": K, "parts": len(parts), "group": G, "call_parts": P,
+           "ops": ops, "sssp": sssp}
+    for mode in modes:
+        out[mode + "_ms"] = round(1e3 * res[mode][0], 2)
+        out[mode + "_GOp_s"] = round(ops / res[mode][0] / 1e9, 1)
+    for mode in modes[1:]:
+        out[mode + "_over_plain"] = round(res["plain"][0] / res[mode][0], 2)
+    out["identical"] = all(res[m][1] == sums0 for m in modes)
+    return out
+
+
+def bench_grid_batch(args):
+    """c3 alone: plain vs batched vs pipelined per-part solve on the grid."""
+    out = grid_section(args, grid_setup(args), modes=("plain", "batched", "pipelined"))
+    res = {"metric": "min-plus GOp/s", "value": out["pipelined_GOp_s"], "unit": "GOp/s",
+           "n_gpus": 1, "steps": args.grid_steps, "warmup": 1,
+           "ms_per_step": out["pipelined_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": out["workload"] + ", pipelined groups"}, "grid_c3": out}
+    print(json.dumps(res), flush=True)
+    return 0 if out["identical"] else 1
 
 
 def _serve_pool(count, V_res, seed):
@@ -507,71 +625,70 @@ def _serve_summary(recs, wall, pool):
             "mean_wave": round(float(np.mean([r[1]["wave_size"] for r in recs])), 2)}
 
 
-def bench_serve(args):
-    """Served solves: Poisson arrivals of whole-graph APSP requests and point
-    query batches against a resident 512x512 road-like grid, admitted by an
-    HBM budget.  First a burst measures the saturation rate; then the same
-    request mix arrives at a fixed rate (--rate-frac of saturation) on the
-    continuous-batching server (queries through the index over
-    --query-parts parts) and on a plain one (one lane, one request per
-    launch sequence, an SSSP per query batch).  Results of both runs must
-    be identical."""
+def serve_section(args, grid):
+    """c5: served solves at a fixed arrival rate.  A burst of the whole mix
+    first measures the saturation rate (reported, not used); then the mix
+    arrives as a Poisson stream at --rate req/s on the continuous-batching
+    server (queries through the index over --query-parts parts) and on a
+    plain one (one lane, one request per wave, an SSSP per query batch).
+    Results of the three runs must be identical."""
     import fastapsp_amd as fa
-    from fastapsp_amd import synth
-    R = args.grid
-    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
-    ro, col, wt = synth.to_csr(R * R, u, v, w, False)
-    pool = _serve_pool(args.requests, R * R, 5)
-    log(f"serve: {len(pool)} requests ({sum(p[0] == 'graph' for p in pool)} graphs), "
-        f"resident V={R * R}")
+    ro, col, wt, V = grid["ro"], grid["col"], grid["wt"], grid["V"]
+    pool = _serve_pool(args.requests, V, 5)
+    log(f"[bench] serve: {len(pool)} requests ({sum(p[0] == 'graph' for p in pool)} graphs), "
+        f"resident V={V}, rate {args.rate} req/s")
     budget = int(args.hbm_gib * (1 << 30))
-    out = {}
+    lam = float(args.rate)
+    gaps = np.random.default_rng(11).exponential(1.0 / lam, len(pool))
+    arrivals = np.cumsum(gaps) - gaps[0]
     with fa.Server(hbm_budget=budget, lanes=args.lanes, query_parts=args.query_parts) as srv:
         t0 = time.perf_counter()
         srv.attach_graph(ro, col, wt)
         attach_s = time.perf_counter() - t0
         ist = srv.stats()
-        log(f"  attach {attach_s:.2f} s (index {ist['index_bytes'] / 2**30:.1f} GiB, "
+        log(f"[bench]   attach {attach_s:.2f} s (index {ist['index_bytes'] / 2**30:.1f} GiB, "
             f"{ist['index_build_ms']:.0f} ms)")
         _serve_run(srv, pool[:16], None)  # warm-up
         recs, wall = _serve_run(srv, pool, None)
         sat = _serve_summary(recs, wall, pool)
         ref = [r[2] for r in recs]
-        lam = args.rate_frac * sat["req_per_s"]
-        log(f"  saturation {sat['req_per_s']} req/s; arrivals at {lam:.1f} req/s")
-        gaps = np.random.default_rng(11).exponential(1.0 / lam, len(pool))
-        arrivals = np.cumsum(gaps) - gaps[0]
         recs, wall = _serve_run(srv, pool, arrivals)
         served = _serve_summary(recs, wall, pool)
         same = [r[2] for r in recs] == ref
         st = srv.stats()
-    log(f"  served: {served}")
+    log(f"[bench]   served: {served}")
     with fa.Server(hbm_budget=budget, lanes=1, max_batch=1) as srv:
         srv.attach_graph(ro, col, wt)
         _serve_run(srv, pool[:16], None)
         recs, wall = _serve_run(srv, pool, arrivals)
         plain = _serve_summary(recs, wall, pool)
         same = same and [r[2] for r in recs] == ref
-    log(f"  plain: {plain}")
-    out = {
-        "metric": "served p50/p99 solve latency at a fixed arrival rate", "value": served["p99_ms"],
-        "unit": "ms (p99)", "n_gpus": 1, "steps": len(pool), "warmup": 16,
-        "ms_per_step": round(1e3 * served["span_s"] / len(pool), 3), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"Poisson stream, half RGG APSP (n logU[256,4096], deg 6, "
-                               f"w 1..255), half 1-64 point queries on a {R}x{R} grid "
-                               f"(V={R * R}); HBM budget {args.hbm_gib} GiB, "
-                               f"{args.lanes} lanes",
-                   "arrival_rate_req_s": round(lam, 2), "rate_frac_of_saturation": args.rate_frac},
-        "saturation": sat, "served": served, "plain_same_rate": plain,
-        "served_over_plain_p99": round(plain["p99_ms"] / served["p99_ms"], 2),
-        "identical": bool(same),
-        "server_stats": {k: st[k] for k in ("waves", "max_wave", "arena_peak", "index_bytes",
-                                            "index_build_ms")},
-        "query_index": {"parts": args.query_parts, "attach_s": round(attach_s, 3)},
-    }
-    print(json.dumps(out), flush=True)
-    return 0 if same else 1
+    log(f"[bench]   plain: {plain}")
+    return {"workload": f"Poisson stream at {lam:g} req/s, half RGG APSP (n logU[256,4096], "
+                        f"deg 6, w 1..255), half 1-64 point queries on the {grid['R']}x"
+                        f"{grid['R']} grid (V={V}); HBM budget {args.hbm_gib} GiB, "
+                        f"{args.lanes} lanes",
+            "arrival_rate_req_s": lam, "requests": len(pool),
+            "p50_ms": served["p50_ms"], "p99_ms": served["p99_ms"],
+            "plain_p50_ms": plain["p50_ms"], "plain_p99_ms": plain["p99_ms"],
+            "served_over_plain_p99": round(plain["p99_ms"] / served["p99_ms"], 2),
+            "served": served, "plain_same_rate": plain, "saturation_burst": sat,
+            "identical": bool(same),
+            "server_stats": {k: st[k] for k in ("waves", "max_wave", "arena_peak", "index_bytes",
+                                                "index_build_ms")},
+            "query_index": {"parts": args.query_parts, "attach_s": round(attach_s, 3)}}
+
+
+def bench_serve(args):
+    """c5 alone."""
+    out = serve_section(args, grid_setup(args))
+    res = {"metric": "served p50/p99 solve latency at a fixed arrival rate", "value": out["p99_ms"],
+           "unit": "ms (p99)", "n_gpus": 1, "steps": out["requests"], "warmup": 16,
+           "ms_per_step": round(1e3 * out["served"]["span_s"] / out["requests"], 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic", "config": {"workload": out["workload"]}, "serve_c5": out}
+    print(json.dumps(res), flush=True)
+    return 0 if out["identical"] else 1
 
 
 def bench_c1(args):
@@ -749,20 +866,18 @@ def bench_sssp_sweep(args):
     return 0
 
 
-def bench_batch(args):
-    """Uneven batch: 512 random geometric graphs, n log-uniform in [64, 2048]
-    (points in the unit square, radius for average degree 6, both arcs with
-    independent weights 1..255), each closed to full APSP (K = 1: dense build
-    + blocked FW).  Plain = one graph per launch sequence, looped on the host
-    (the reference's structure); batched = one phase-synchronous launch
-    sequence over all graphs (size-ordered task queues, paired pivot
-    rounds), replayed from a hipGraph.  --batched-only skips the plain loop
-    (its ~28K launches make counter runs slow)."""
+def batch_section(args, ctx, lib, ceiling):
+    """c4: 512 random geometric graphs, n log-uniform in [64, 2048] (points in
+    the unit square, radius for average degree 6, both arcs with independent
+    weights 1..255), each closed to full APSP as one part (K = 1: the
+    reference's pipeline with no boundary vertices, so its SSSP stage is
+    empty -- dense build + blocked FW).  Plain = one graph per launch
+    sequence, looped on the host (fap/fap.cpp:189-193); batched = one
+    phase-synchronous launch sequence over all graphs (size-ordered task
+    queues, paired pivot rounds, two chains), replayed from a hipGraph.
+    Checksums of all matrices must agree."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
-    lib = fa.load()
-    ctx = fa.Context(0)
-    ceiling = ctx.probe_minplus_ceiling()
     sizes = synth.loguniform_sizes(args.jobs, 64, 2048, 4).astype(int)
     graphs, ptrs, lds = [], [], []
     ld_of = [(n + 3) // 4 * 4 for n in sizes]
@@ -797,9 +912,10 @@ def bench_batch(args):
     runs = (("batched", batched),) if args.batched_only else (("plain", plain), ("batched", batched))
     for name, fn in runs:
         build()
+        fn()
         ctx.sync()
         t_ms = []
-        for _ in range(args.steps):
+        for _ in range(args.batch_steps):
             build()
             ctx.sync()
             t0 = time.perf_counter()
@@ -808,43 +924,61 @@ def bench_batch(args):
             t_ms.append(1e3 * (time.perf_counter() - t0))
         res[name] = float(np.median(t_ms))
         res[name + "_sum"] = ctx.checksum(pool.ptr, total)
-        log(f"[bench] {name}: {res[name]:.2f} ms")
-    same = None if args.batched_only else res["plain_sum"] == res["batched_sum"]
-    plain_ms = res.get("plain")
-    out = {
-        "metric": "min-plus GOp/s", "value": round(ops / (res["batched"] * 1e-3) / 1e9, 1),
-        "unit": "GOp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
-        "ms_per_step": round(res["batched"], 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"{args.jobs} random geometric graphs, n log-uniform in "
-                               f"[64, 2048], deg 6, w 1..255, K=1 closure",
-                   "jobs": int(args.jobs), "sum_n": int(sizes.sum()), "ops": ops},
-        "plain": None if plain_ms is None else {
-            "ms": round(plain_ms, 3), "GOp/s": round(ops / (plain_ms * 1e-3) / 1e9, 1)},
-        "batched": {"ms": round(res["batched"], 3),
-                    "GOp/s": round(ops / (res["batched"] * 1e-3) / 1e9, 1),
-                    "frac_of_ceiling": round(ops / (res["batched"] * 1e-3) / 1e12 / ceiling, 3),
-                    "frac_of_spec_peak": round(ops / (res["batched"] * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3)},
-        "minplus_ceiling_TOPs": round(ceiling, 2),
-        "batched_launches": info["launches"], "batched_tasks": info["tasks"],
-        "speedup_batched_vs_plain": None if plain_ms is None else round(plain_ms / res["batched"], 2),
-        "plain_and_batched_identical": same,
-        "tuned": tuned,
-    }
-    print(json.dumps(out), flush=True)
+        log(f"[bench] c4 {name}: {res[name]:.2f} ms")
     plan.close()
     for g in graphs:
         g.close()
-    return 1 if same is False else 0
+    pool.free()
+    same = None if args.batched_only else res["plain_sum"] == res["batched_sum"]
+    plain_ms = res.get("plain")
+    bt = res["batched"]
+    return {"workload": f"{args.jobs} random geometric graphs, n log-uniform in [64, 2048], "
+                        f"deg 6, directed, w 1..255; full APSP per graph as one part (K=1: "
+                        f"dense build + blocked FW, no boundary so no SSSP stage)",
+            "jobs": int(args.jobs), "sum_n": int(sizes.sum()), "ops": ops,
+            "plain_ms": None if plain_ms is None else round(plain_ms, 3),
+            "batched_ms": round(bt, 3),
+            "plain_GOp_s": None if plain_ms is None else round(ops / (plain_ms * 1e-3) / 1e9, 1),
+            "batched_GOp_s": round(ops / (bt * 1e-3) / 1e9, 1),
+            "batched_over_plain": None if plain_ms is None else round(plain_ms / bt, 2),
+            "batched_frac_of_ceiling": round(ops / (bt * 1e-3) / 1e12 / ceiling, 3),
+            "batched_frac_of_spec_peak": round(ops / (bt * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3),
+            "batched_launches": info["launches"], "batched_tasks": info["tasks"],
+            "steps": args.batch_steps, "identical": same, "tuned": tuned}
+
+
+def bench_batch(args):
+    """c4 alone."""
+    import fastapsp_amd as fa
+    lib = fa.load()
+    ctx = fa.Context(0)
+    ceiling = ctx.probe_minplus_ceiling()
+    out = batch_section(args, ctx, lib, ceiling)
+    ctx.close()
+    res = {"metric": "min-plus GOp/s", "value": out["batched_GOp_s"], "unit": "GOp/s",
+           "n_gpus": 1, "steps": args.batch_steps, "warmup": 1,
+           "ms_per_step": out["batched_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": out["workload"]}, "minplus_ceiling_TOPs": round(ceiling, 2),
+           "batch_c4": out}
+    print(json.dumps(res), flush=True)
+    return 1 if out["identical"] is False else 0
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--cpu-rows", type=int, default=24)
+    ap.add_argument("--cpu-rows", type=int, default=16,
+                    help="rows of the product timed for the CPU baseline (0: none)")
+    ap.add_argument("--cpu-fw-n", type=int, default=1024,
+                    help="leading block closed by the CPU FW for the baseline")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="default workload: only the dense step (no c4/c5/c3 sections)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--grid-steps", type=int, default=1)
     ap.add_argument("--workload", default="dense",
                     choices=["dense", "copy", "c1", "grid", "gridbatch", "fwsweep",
                              "ssspsweep", "batch", "serve"])
@@ -858,7 +992,8 @@ def main():
     ap.add_argument("--group", type=int, default=32)
     ap.add_argument("--call-parts", type=int, default=128)
     ap.add_argument("--requests", type=int, default=400)
-    ap.add_argument("--rate-frac", type=float, default=0.5)
+    ap.add_argument("--rate", type=float, default=2500.0,
+                    help="served workload: fixed Poisson arrival rate, requests per second")
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--hbm-gib", type=float, default=16.0)
     ap.add_argument("--query-parts", type=int, default=256,

@@ -216,6 +216,12 @@ int fapg_get_subgraph_distance_i32(fapg *h, int32_t id, int32_t *out) {
   return copy_out(v, out, "distance");
 }
 
+int fapg_get_subgraph_rows_i32(fapg *h, int32_t id, int32_t row0, int32_t nrows, int32_t *out) {
+  FAPG_CHECK(h);
+  if (h->g->getSubGraphRowsInt32(id, row0, nrows, out) != 0) return from_status(h->g);
+  return FAP_OK;
+}
+
 int fapg_get_subgraph_distance_f32(fapg *h, int32_t id, float *out) {
   FAPG_CHECK(h);
   std::vector<float> v = h->g->getSubGraphDistance(id);

@@ -614,6 +614,23 @@ std::vector<int32_t> fapGraph::getSubGraphPath(int32_t id) {
   return out;
 }
 
+int32_t fapGraph::getSubGraphRowsInt32(int32_t id, int32_t row0, int32_t nrows, int32_t *out) {
+  reset_status();
+  if (!part_ok(id) || row0 < 0 || nrows < 0 || (int64_t)row0 + nrows > C_BlockVer_num[id] ||
+      (nrows > 0 && !out))
+    return fail(FAP_E_ARG, "row range outside the part");
+  int64_t ld = 0;
+  const int32_t *p = subGraphDistanceDevice(id, &ld);
+  if (!p) return fail(FAP_E_STATE, "part " + std::to_string(id) + " has not been solved");
+  for (int32_t r = 0; r < nrows; r++)
+    if (fap_memcpy_d2h(dev_->ctx, out + (int64_t)r * num_vertexs, p + (row0 + r) * ld,
+                       sizeof(int32_t) * num_vertexs) != FAP_OK)
+      return fail(FAP_E_HIP, fap_last_error());
+  status_ = 0;
+  error_.clear();
+  return 0;
+}
+
 std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
   reset_status();
   if (v < 0 || v >= num_vertexs || !part_ok(current_subgraph_id) ||

@@ -159,6 +159,12 @@ class Graph:
               "getSubGraphDistance")
         return out
 
+    def getSubGraphRowsInt32(self, sub_graph_id: int, row0: int, nrows: int) -> np.ndarray:
+        out = np.empty((nrows, self.num_vertexs), np.int32)
+        check(load().fapg_get_subgraph_rows_i32(self.handle, int(sub_graph_id), int(row0),
+                                                int(nrows), _p32(out)), "getSubGraphRows")
+        return out
+
     def getSubGraphDistance(self, sub_graph_id: int) -> np.ndarray:
         n = int(self.part_info()[0][sub_graph_id])
         out = np.empty((n, self.num_vertexs), np.float32)

@@ -157,6 +157,9 @@ class fapGraph {
   // int32 results (INF = FAP_INF)
   std::vector<int32_t> getAllDistanceInt32();
   std::vector<int32_t> getSubGraphDistanceInt32(int32_t sub_garph_id);
+  // rows [row0, row0 + nrows) of a solved part's block (row r = vertex
+  // st2ed[offset + r]) into out (nrows x V); 0 or -1
+  int32_t getSubGraphRowsInt32(int32_t sub_garph_id, int32_t row0, int32_t nrows, int32_t *out);
   // device-resident view of a solved block: row r of part id at ptr + r*ld
   const int32_t *subGraphDistanceDevice(int32_t sub_garph_id, int64_t *ld);
   // distance between two original vertex ids, from the resident result

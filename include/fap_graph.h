@@ -77,6 +77,9 @@ int fapg_get_subgraph_distance_f32(fapg *g, int32_t id, float *out);
 /* V x V after fapg_solve */
 int fapg_get_all_distance_i32(fapg *g, int32_t *out);
 int fapg_get_all_distance_f32(fapg *g, float *out);
+/* rows [row0, row0 + nrows) of a solved part (st2ed order), nrows x V int32;
+ * no counterpart in the reference (its getters copy whole blocks) */
+int fapg_get_subgraph_rows_i32(fapg *g, int32_t id, int32_t row0, int32_t nrows, int32_t *out);
 /* row of vertex v in the current part (V floats) */
 int fapg_get_distance_from_one_point(fapg *g, int32_t v, float *out);
 int fapg_distance_p2p(fapg *g, int32_t src, int32_t dst, int32_t *out);
