@@ -88,6 +88,7 @@ _SIGS = {
     "fap_sssp_multi": (_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp]),
     "fap_fw_blocked": (_int, [_vp, _vp, _i32, _i64, _vp]),
     "fap_fw_batched": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _vp]),
+    "fap_fw_batched_noalloc": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _vp]),
     "fap_fw_plan_create": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _int,
                                   C.POINTER(_vp)]),
     "fap_fw_plan_run": (_int, [_vp, _vp]),

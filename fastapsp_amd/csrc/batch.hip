@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -403,6 +404,37 @@ int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
   return FAP_OK;
 }
 
+// Scratch of fap_fw_batched_noalloc: the batch's job and segment
+// descriptors, in a file-scope device array instead of a hipMalloc'd
+// buffer.  Each device holds its own copy until the process ends.  In host
+// code `noalloc_scratch` names a host-side stand-in, never the device
+// memory: every host use goes through the address hipGetSymbolAddress
+// returned for the current device (noalloc_scratch_ptr).
+__device__ __align__(256) char noalloc_scratch[FAP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `noalloc_scratch` on `device` (made current by the
+// caller), looked up once per device.
+static int noalloc_scratch_ptr(int device, char **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(FAP_E_ARG, "no-alloc scratch: device id " + std::to_string(device) +
+                               " beyond the " + std::to_string(kMaxDevices) + " supported");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    FAP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch)));
+    if (!ptr) return fail(FAP_E_HIP, "no-alloc scratch: null symbol address");
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = (char *)g_scratch_ptr[device];
+  return FAP_OK;
+}
+
 static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, const int64_t *ld) {
   if (count < 0 || (count > 0 && (!d_mats || !n || !ld)))
     return fail(FAP_E_ARG, "fw_batched: bad arguments");
@@ -438,6 +470,36 @@ int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const in
   }
   fw_plan_destroy(p);
   return rc;
+}
+
+int fap_fw_batched_noalloc(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                           const int64_t *ld, void *stream) {
+  if (!ctx) return fail(FAP_E_ARG, "ctx is null");
+  FAP_TRY(check_jobs(count, d_mats, n, ld));
+  if (count == 0) return FAP_OK;
+  FAP_HIP(hipSetDevice(ctx->device));
+  FwBatchDesc d;
+  build_fw_batch(count, d_mats, n, ld, &d);
+  const size_t jobs_b = sizeof(fapk::BatchJob) * d.jobs.size();
+  const size_t segs_off = (jobs_b + 255) & ~(size_t)255;
+  const size_t segs_b = sizeof(fapk::BatchSeg) * d.segs.size();
+  if (segs_off + segs_b > (size_t)FAP_NOALLOC_SCRATCH_BYTES)
+    return fail(FAP_E_ARG, "fw_batched_noalloc: the batch's descriptors need " +
+                               std::to_string(segs_off + segs_b) + " bytes, above the " +
+                               std::to_string((long long)FAP_NOALLOC_SCRATCH_BYTES) +
+                               "-byte scratch (fewer or smaller matrices per call)");
+  char *scratch = nullptr;
+  FAP_TRY(noalloc_scratch_ptr(ctx->device, &scratch));
+  hipStream_t s = pick_stream(ctx, stream);
+  // the copies are ordered after earlier work on s, so an earlier call on
+  // the same stream has finished reading the scratch before they land
+  if (jobs_b) FAP_HIP(hipMemcpyAsync(scratch, d.jobs.data(), jobs_b, hipMemcpyHostToDevice, s));
+  if (segs_b)
+    FAP_HIP(hipMemcpyAsync(scratch + segs_off, d.segs.data(), segs_b, hipMemcpyHostToDevice, s));
+  FAP_TRY(launch_fw_batch(ctx, d, (const fapk::BatchJob *)scratch,
+                          (const fapk::BatchSeg *)(scratch + segs_off), s));
+  FAP_HIP(hipStreamSynchronize(s));
+  return FAP_OK;
 }
 
 int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,

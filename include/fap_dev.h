@@ -149,6 +149,19 @@ int fap_fw_blocked(fap_ctx *ctx, int32_t *d_mat, int32_t n, int64_t ld,
 int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                    const int64_t *ld, void *stream);
 
+/* fap_fw_batched without device allocation, for callers that cannot call
+ * hipMalloc (code under stream capture, tight serving loops, harnesses that
+ * forbid it).  No counterpart in the reference.  The batch's descriptors
+ * (24 bytes per matrix + 16 per phase segment) go to a fixed device array
+ * of FAP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a
+ * batch that needs more is refused with FAP_E_ARG.  One chain of launches on
+ * `stream`, same results as fap_fw_batched.  Calls that overlap on one GPU
+ * (other threads, other streams) share that array and race; calls on one
+ * stream are ordered.  Synchronous. */
+#define FAP_NOALLOC_SCRATCH_BYTES (4 << 20)
+int fap_fw_batched_noalloc(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                           const int64_t *ld, void *stream);
+
 /* A reusable batch: descriptors uploaded once; with use_graph the phase
  * launches are captured in a hipGraph and each run replays it (the
  * matrices are read and written in place at every run).  A plan of 8 or
