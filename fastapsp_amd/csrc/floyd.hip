@@ -10,9 +10,10 @@
 //      (I, r) = min(C, C (x) D_rr) for all I, J != r: one launch;
 //   3. every other tile (I, J) = min(C, D_Ir (x) D_rJ): one launch.
 // Phases 2 and 3 reuse the register-blocked min-plus tile (minplus_tile.h)
-// with T x T tiles.  T = 128 halves the number of rounds (less HBM traffic
-// per closure); T = 64 gives 4x more tiles per launch and a cheap diagonal
-// phase, and is the default (faster at every measured n).
+// with T x T tiles.  T = 128 halves the number of rounds and, with 8 x 8
+// cells per lane, the LDS bytes per update (phase 3 is bound by LDS reads
+// at 4 x 4 cells); T = 64 gives 4x more tiles per launch and a cheap
+// diagonal phase.  128 is the default for large n, 64 below (launch_fw).
 #include "common.h"
 #include "minplus_tile.h"
 
@@ -109,6 +110,71 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
+// Closure of a 128 x 128 diagonal block (kb <= 128 valid rows/columns) by
+// one 256-thread workgroup, in global memory: the 2 x 2 recursion of
+// fw_diag_kernel<128> with each 64-block closed in LDS (close_block64) and
+// each 64 x 64 product done by the product tile (C may alias A or B: the
+// tile reads its operands before it writes).  `lds` holds BLK_LDS_WORDS.
+__device__ __forceinline__ void close_block128(int32_t *blk, int64_t ld, int kb, int flags,
+                                               int32_t *lds) {
+  const int n0 = imin(64, kb), n1 = kb - 64;
+  close_block64(blk, ld, n0, lds);
+  __syncthreads();
+  if (n1 <= 0) return;
+  int32_t *d00 = blk, *d01 = blk + 64, *d10 = blk + 64 * ld, *d11 = blk + 64 * ld + 64;
+  MpSmall::run(d00, ld, d01, ld, d01, ld, n0, n1, n0, true, flags, lds);  // D01 (+)= D00 D01
+  __syncthreads();
+  MpSmall::run(d10, ld, d00, ld, d10, ld, n1, n0, n0, true, flags, lds);  // D10 (+)= D10 D00
+  __syncthreads();
+  MpSmall::run(d10, ld, d01, ld, d11, ld, n1, n1, n0, true, flags, lds);  // D11 (+)= D10 D01
+  __syncthreads();
+  close_block64(d11, ld, n1, lds);
+  __syncthreads();
+  MpSmall::run(d11, ld, d10, ld, d10, ld, n1, n0, n1, true, flags, lds);  // D10 (+)= D11 D10
+  __syncthreads();
+  MpSmall::run(d01, ld, d11, ld, d01, ld, n0, n1, n1, true, flags, lds);  // D01 (+)= D01 D11
+  __syncthreads();
+  MpSmall::run(d01, ld, d10, ld, d00, ld, n0, n0, n1, true, flags, lds);  // D00 (+)= D01 D10
+  __syncthreads();
+}
+
+// Phase 2 for T = 128 in 64 x 64 pieces: each panel tile (r, J) or (I, r)
+// is split into four 64 x 64 output quarters, one workgroup each (K = 128,
+// 4 x 4 cells per lane), so the phase runs 4 (2 nb - 2) short workgroups
+// instead of 2 nb - 2 long ones (n = 8192: 19 us per round against 40).
+// A quarter reads the other row (column) half of its own panel tile while a
+// sibling writes it; every value it can see there is the length of a path
+// through pivot block r no shorter than the finished one, so the minimum it
+// forms is the same either way.  (Staging the whole 128-deep operands in
+// LDS at once, 67 KB per workgroup, measured 21 us.)
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void fw_panel128_kernel(
+    int32_t *d, int n, int64_t ld, int r, int nb, int flags) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  const int base = r * 128;
+  const int kb = imin(128, n - base);
+  const int32_t *diag = d + (int64_t)base * ld + base;
+  const int q = blockIdx.x & 3, hr = q >> 1, hc = q & 1;
+  int bid = blockIdx.x >> 2;
+  if (bid < nb - 1) {
+    const int J = bid < r ? bid : bid + 1;
+    const int rows = kb - 64 * hr, cols = imin(128, n - J * 128) - 64 * hc;
+    if (rows <= 0 || cols <= 0) return;
+    int32_t *c = d + (int64_t)base * ld + (int64_t)J * 128;
+    MpSmall::run(diag + (int64_t)(64 * hr) * ld, ld, c + 64 * hc, ld,
+                 c + (int64_t)(64 * hr) * ld + 64 * hc, ld, imin(64, rows), imin(64, cols), kb,
+                 true, flags, lds);
+  } else {
+    bid -= nb - 1;
+    const int I = bid < r ? bid : bid + 1;
+    const int rows = imin(128, n - I * 128) - 64 * hr, cols = kb - 64 * hc;
+    if (rows <= 0 || cols <= 0) return;
+    int32_t *c = d + (int64_t)I * 128 * ld + base;
+    MpSmall::run(c + (int64_t)(64 * hr) * ld, ld, diag + 64 * hc, ld,
+                 c + (int64_t)(64 * hr) * ld + 64 * hc, ld, imin(64, rows), imin(64, cols), kb,
+                 true, flags, lds);
+  }
+}
+
 // Phase 3 over a set of tiles, selected by `mode` (p = r + 1):
 //   0: every tile outside row r and column r, (nb-1)^2 of them;
 //   1: the look-ahead tiles, row p and column p minus row/column r (2nb-3);
@@ -117,17 +183,17 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 // diagonal and panels (which only touch row/column p) start while mode 2
 // runs.
 //
-// With `fuse` (T = 64, mode 0, r + 1 < nb) workgroup 0 takes tile
-// (r+1, r+1) -- swapping with the workgroup that had it -- and after its
-// update closes it in LDS: the next round's diagonal phase, done while the
-// other tiles of this phase run, so that round launches only its panels.
+// With `fuse` (mode 0, r + 1 < nb) workgroup 0 takes tile (r+1, r+1) --
+// swapping with the workgroup that had it -- and after its update closes
+// it (in LDS for T = 64, by close_block128 for T = 128): the next round's
+// diagonal phase, done while the other tiles of this phase run, so that
+// round launches only its panels.
 template <int TM>
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void fw_rest_kernel(
     int32_t *d, int n, int64_t ld, int r, int nb, int flags, int mode, int fuse) {
   using Tile = MpTile<TM, TM>;
   constexpr int T = Tile::BM;
-  constexpr int LDS_WORDS = (TM == 4 && BLK_LDS_WORDS > Tile::LDS_WORDS) ? BLK_LDS_WORDS
-                                                                         : Tile::LDS_WORDS;
+  constexpr int LDS_WORDS = BLK_LDS_WORDS > Tile::LDS_WORDS ? BLK_LDS_WORDS : Tile::LDS_WORDS;
   __shared__ __attribute__((aligned(16))) int32_t lds[LDS_WORDS];
   int I, J;
   if (mode == 0) {
@@ -167,9 +233,10 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   const int32_t *a = d + (int64_t)I * T * ld + base;
   const int32_t *b = d + (int64_t)base * ld + (int64_t)J * T;
   Tile::run(a, ld, b, ld, c, ld, imin(T, n - I * T), imin(T, n - J * T), kb, true, flags, lds);
-  if (TM == 4 && fuse && blockIdx.x == 0) {
+  if (fuse && blockIdx.x == 0) {
     __syncthreads();  // the tile's writes above come before the closure reads it
-    close_block64(c, ld, imin(T, n - I * T), lds);
+    if (TM == 4) close_block64(c, ld, imin(T, n - I * T), lds);
+    else close_block128(c, ld, imin(T, n - I * T), flags, lds);
   }
 }
 
@@ -194,8 +261,12 @@ struct Fw {
       FAP_CHECK_LAUNCH();
     }
     if (nb > 1) {
-      hipLaunchKernelGGL(fapk::fw_panel_kernel<TM>, dim3(2 * (nb - 1)), dim3(fapk::MP_THREADS),
-                         0, s, d, n, ld, r, nb, flags);
+      if constexpr (TM == 8)
+        hipLaunchKernelGGL(fapk::fw_panel128_kernel, dim3(8 * (nb - 1)), dim3(fapk::MP_THREADS),
+                           0, s, d, n, ld, r, nb, flags);
+      else
+        hipLaunchKernelGGL(fapk::fw_panel_kernel<TM>, dim3(2 * (nb - 1)),
+                           dim3(fapk::MP_THREADS), 0, s, d, n, ld, r, nb, flags);
       FAP_CHECK_LAUNCH();
     }
     return FAP_OK;
@@ -222,8 +293,11 @@ struct Fw {
     const int flags = vec_flags(d, ld, d, ld, d, ld);
     FAP_TRY(diag_panel(d, n, ld, 0, nb, flags, s));
     const bool overlap = lookahead && ctx && ctx->aux_stream && nb >= 3;
-    const bool fuse_ok =
-        TM == 4 && (int64_t)(nb - 1) * (nb - 1) >= (int64_t)(ctx ? ctx->num_cus : 256) * 8;
+    // fuse the next diagonal closure into phase 3 when the phase is long
+    // enough to hide it: at least 8 tiles per CU for T = 64 (a 256-thread
+    // 64-closure), 4 for T = 128 (the 128-closure's seven steps)
+    const int64_t cus = ctx ? ctx->num_cus : 256;
+    const bool fuse_ok = (int64_t)(nb - 1) * (nb - 1) >= cus * (TM == 4 ? 8 : 4);
     for (int r = 0; r < nb; r++) {
       if (r + 1 < nb && overlap) {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 1, s));
@@ -234,10 +308,10 @@ struct Fw {
         FAP_TRY(rest(d, n, ld, r, nb, flags, 2, s));
         FAP_HIP(hipStreamWaitEvent(s, ctx->ev_fw_panel, 0));
       } else {
-        // T = 64: round r's phase 3 also closes the next diagonal block when
-        // the phase is long enough to hide a 256-thread closure (at least 8
-        // tiles per CU; below that the 1024-thread diagonal kernel is
-        // faster: n = 1024 took 0.56 ms unfused, 0.64 ms fused)
+        // round r's phase 3 also closes the next diagonal block when the
+        // phase is long enough to hide it (below that the 1024-thread
+        // diagonal kernel is faster: T = 64, n = 1024 took 0.56 ms unfused,
+        // 0.64 ms fused)
         const int fuse = (fuse_ok && r + 1 < nb) ? 1 : 0;
         FAP_TRY(rest(d, n, ld, r, nb, flags, 0, s, fuse));
         if (r + 1 < nb) FAP_TRY(diag_panel(d, n, ld, r + 1, nb, flags, s, !fuse));
@@ -250,9 +324,14 @@ struct Fw {
 }  // namespace
 
 // Block size and look-ahead: the context options if set, else the tune
-// cache, else 64 without look-ahead.  Measured on MI355X, 64 beats 128 from
-// n = 256 to n = 8192: 4x more tiles per launch and a diagonal phase that
-// is 8x cheaper outweigh twice the rounds.
+// cache, else 128 when phase 3 has at least 8 tiles of 128 per CU and 64
+// below, without look-ahead (fw_settings).  Measured on MI355X (closure
+// ms, T = 64 / 128): n = 8192 20.7 / 19.0, 6144 9.49 / 8.86, 5120 5.99 /
+// 6.54, 4096 3.59 / 4.87, 2048 1.40 / 1.83 -- the 128 x 128 tile with 8 x 8
+// cells per lane halves the LDS bytes per update of the 64 x 64 tile's
+// 4 x 4 cells, and with the next diagonal closed inside phase 3 and the
+// panels split in quarters, the serial phases of T = 128 no longer cost
+// more than that saves once the grid is large.
 int launch_fw(fap_ctx *ctx, int32_t *d, int32_t n, int64_t ld, hipStream_t s) {
   if (n <= 0) return FAP_OK;
   int tile;

@@ -75,9 +75,14 @@ fap_tune_cache *tune_env_cache() {
 void fw_settings(const fap_ctx *ctx, int32_t n, int *tile, bool *lookahead) {
   int32_t p[2] = {64, 0};
   const bool cached = tune_lookup(ctx, "fw", fap_tune_key_fw(n), p);
+  // default: 128 once phase 3 has at least 8 tiles of 128 x 128 per CU
+  // (n >= ~6,000 on 256 CUs), where it measured faster; 64 below
+  const int64_t nb128 = ((int64_t)n + 127) / 128;
+  const int def_tile =
+      (nb128 - 1) * (nb128 - 1) >= 8 * (int64_t)(ctx ? ctx->num_cus : 256) ? 128 : 64;
   *tile = (ctx && (ctx->opt_fw_tile == 64 || ctx->opt_fw_tile == 128)) ? ctx->opt_fw_tile
           : cached                                                     ? p[0]
-                                                                       : 64;
+                                                                       : def_tile;
   *lookahead = (ctx && ctx->opt_fw_lookahead >= 0) ? ctx->opt_fw_lookahead != 0
                : cached                            ? p[1] != 0
                                                    : false;
@@ -236,7 +241,15 @@ int fap_tune_cache_load(fap_tune_cache *c, const char *path) {
   return FAP_OK;
 }
 
-int64_t fap_tune_key_fw(int32_t n) { return ceil_log2((uint64_t)(n > 1 ? n : 1)); }
+// Quarter octaves of n: the smallest k with n^4 <= 2^k (n <= 46340, so n^4
+// fits in 64 bits).  The best block size changes between 5,120 and 6,144,
+// one octave bucket would hold both.  A dense closure relaxes every cell in
+// every round, so its cost does not depend on the matrix's density, which is
+// therefore not part of the key.
+int64_t fap_tune_key_fw(int32_t n) {
+  const uint64_t v = (uint64_t)(n > 1 ? n : 1);
+  return ceil_log2(v * v * v * v);
+}
 
 int64_t fap_tune_key_sssp(int32_t V, int64_t E, int32_t n_src) {
   const int64_t vb = ceil_log2((uint64_t)(V > 1 ? V : 1));

@@ -13,10 +13,13 @@
  *
  *   kind     key                                   params
  *   "fw"     fap_tune_key_fw(n)                    {tile 64|128, lookahead 0|1}
+ *            (quarter octaves of n; a dense closure's cost does not depend on
+ *            density, the SSSP key carries the graph's mean degree)
  *   "sssp"   fap_tune_key_sssp(V, E, sources)      {threads 256|512|1024, delta / mean weight}
  *   "batch"  fap_tune_key_batch(count, n)          {workgroups per CU per launch}
  *
- * With no entry the defaults are tile 64, no look-ahead, SSSP threads 1024
+ * With no entry the defaults are tile 128 when ceil(n / 128) - 1 squared is
+ * at least 8 per CU (64 below), no look-ahead, SSSP threads 1024
  * while the sources fit on the CUs (512 up to twice that, 256 beyond),
  * delta 8 x mean weight while the sources fit on the CUs (6 x up to twice
  * that, 4 x beyond), 64 workgroups per CU.
@@ -51,7 +54,7 @@ int fap_tune_cache_set(fap_tune_cache *c, const char *kind, int64_t key, const i
 int fap_tune_cache_get(fap_tune_cache *c, const char *kind, int64_t key, int32_t *params,
                        int32_t nparams, double *ms);
 
-/* buckets: ceil(log2 n); (ceil(log2 V) * 64 + round(2 log2(1 + E / V))) * 32
+/* buckets: ceil(log2 n^4) (quarter octaves); (ceil(log2 V) * 64 + round(2 log2(1 + E / V))) * 32
  * + ceil(log2 sources); ceil(log2 of the batch's 64-tile triple count) */
 int64_t fap_tune_key_fw(int32_t n);
 int64_t fap_tune_key_sssp(int32_t V, int64_t E, int32_t n_src);

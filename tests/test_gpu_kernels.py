@@ -166,6 +166,38 @@ def test_fw_fused_diagonal_matches_unfused(gpu_ctx):
     np.testing.assert_array_equal(ref[rows], want)
 
 
+@pytest.mark.parametrize("n", [4200, 4500])
+def test_fw_tile128_fused_closure_matches_tile64(gpu_ctx, n):
+    """128-wide blocks with at least 4 tiles per CU: each round's remaining-
+    tile launch closes the next 128-block (2 x 2 recursion on 64-blocks in
+    one workgroup) and the panels run as 64 x 64 quarters.  n = 4200 ends
+    with a 104-wide block (both recursion halves), 4500 with a 20-wide one.
+    Equal to the 64-block closure (itself checked against the oracle above),
+    strided, and sampled rows equal Dijkstra rows of the matrix graph."""
+    m = synth.dense_matrix(n, 0.004, 1, 1000, n)
+    out = {}
+    for tile in (64, 128):
+        gpu_ctx.set_option("fw_tile", tile)
+        try:
+            ld = n + 4 * (tile // 64)
+            d = padded(gpu_ctx, m, ld)
+            gpu_ctx.fw(d.ptr, n, ld)
+            got = d.get().reshape(n, ld)
+            d.free()
+        finally:
+            gpu_ctx.set_option("fw_tile", 0)
+        assert (got[:, n:] == -7).all()
+        out[tile] = got[:, :n]
+    np.testing.assert_array_equal(out[128], out[64])
+    rows = np.array([0, 127, 128, n // 2, n - 1])
+    mask = (m < O.INF) & ~np.eye(n, dtype=bool)
+    src, dst = np.nonzero(mask)
+    ro = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(src, minlength=n), out=ro[1:])
+    want = O.batched_sssp(rows, ro, dst.astype(np.int32), m[src, dst].astype(np.int32))
+    np.testing.assert_array_equal(out[128][rows], want)
+
+
 def test_fw_golden_and_strided(gpu_ctx):
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dense70.npz"))

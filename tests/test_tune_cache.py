@@ -12,7 +12,9 @@ from fastapsp_amd import TuneCache
 
 def test_bucket_keys():
     kf = TuneCache.key_fw
-    assert [kf(n) for n in (1, 2, 3, 64, 65, 8192, 8193)] == [0, 1, 2, 6, 7, 13, 14]
+    # quarter octaves: smallest k with n^4 <= 2^k
+    assert [kf(n) for n in (1, 2, 3, 64, 65, 4096, 5120, 6144, 8192, 8193)] == \
+        [0, 4, 7, 24, 25, 48, 50, 51, 52, 53]
     # (V bucket * 64 + round(2 log2(1 + E/V))) * 32 + sources bucket
     db = round(2 * np.log2(1 + 952362 / 262144))
     assert TuneCache.key_sssp(262144, 952362, 74) == (18 * 64 + db) * 32 + 7
