@@ -146,6 +146,8 @@ _SIGS = {
     "fap_minplus_path": (_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
                                 _i32, _i32, _i32, _vp]),
     "fap_fw_blocked_path": (_int, [_vp, _vp, _i32, _i64, _vp, _i64, _vp]),
+    "fap_graph_build_index": (_int, [_vp, _vp, C.POINTER(_i32)]),
+    "fap_graph_index_info": (_int, [_vp, C.POINTER(_i32), C.POINTER(_i64)]),
     "fap_subgraph_solve_groups": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, _vp, _i64, _i32]),
     "fap_subgraph_pred": (_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
@@ -519,6 +521,17 @@ class DeviceGraph:
     def subgraph_solve(self, start, n_sub, n_bdy, rowblock, ld_rb, stream=None):
         check(load().fap_subgraph_solve(self.ctx.handle, self.handle, start, n_sub, n_bdy,
                                         rowblock, ld_rb, stream), "fap_subgraph_solve")
+
+    def build_index(self) -> bool:
+        b = _i32()
+        check(load().fap_graph_build_index(self.ctx.handle, self.handle, C.byref(b)),
+              "fap_graph_build_index")
+        return bool(b.value)
+
+    def index_info(self):
+        nb, by = _i32(), _i64()
+        check(load().fap_graph_index_info(self.handle, C.byref(nb), C.byref(by)))
+        return nb.value, by.value
 
     def subgraph_solve_batch(self, starts, n_subs, n_bdys, rowblocks, ld_rb, stream=None):
         k = len(starts)

@@ -201,7 +201,8 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   Workspace *all[] = {&ctx->ws_sub, &ctx->ws_queue, &ctx->ws_stamp, &ctx->ws_small,
-                      &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense, &ctx->ws_srcs};
+                      &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense, &ctx->ws_srcs,
+                      &ctx->ws_bidx};
   for (Workspace *w : all)
     if (w->ptr) (void)hipFree(w->ptr);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
@@ -251,6 +252,12 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
     if (value != 0 && value != 1 && value != 2 && value != 4)
       return fail(FAP_E_ARG, "sssp_arcs must be 0 (auto), 1, 2 or 4");
     ctx->opt_sssp_arcs = value == 0 ? 2 : (int)value;
+  } else if (k == "sssp_index") {
+    if (value < 0 || value > 1) return fail(FAP_E_ARG, "sssp_index must be 0 or 1");
+    ctx->opt_sssp_index = (int)value;
+  } else if (k == "bidx_delta") {
+    if (value < 0 || value > INT32_MAX / 4) return fail(FAP_E_ARG, "bidx_delta out of range");
+    ctx->opt_bidx_delta = (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 64) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 64");
     ctx->opt_batch_grid = (int)value;
@@ -273,6 +280,8 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_pack") *value = ctx->opt_sssp_pack;
   else if (k == "sssp_wgs") *value = ctx->opt_sssp_wgs;
   else if (k == "sssp_arcs") *value = ctx->opt_sssp_arcs;
+  else if (k == "sssp_index") *value = ctx->opt_sssp_index;
+  else if (k == "bidx_delta") *value = ctx->opt_bidx_delta;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }
@@ -492,12 +501,14 @@ int fap_graph_set_partition(fap_ctx *ctx, fap_graph_dev *g, const int32_t *graph
   FAP_HIP(hipStreamSynchronize(ctx->stream));
   g->h_st2ed.assign(st2ed, st2ed + V);
   g->h_graph_id.assign(graph_id, graph_id + V);
+  bidx_release(g);  // an index belongs to the partition it was built for
   return FAP_OK;
 }
 
 int fap_graph_free(fap_ctx *ctx, fap_graph_dev *g) {
   if (!g) return FAP_OK;
   if (ctx) (void)hipSetDevice(ctx->device);
+  bidx_release(g);
   int32_t *bufs[] = {g->row_offset, g->graph_id, g->st2ed, g->ed2st, g->in_offset};
   for (int32_t *b : bufs)
     if (b) (void)hipFree(b);

@@ -105,6 +105,51 @@ struct MpTile {
       *reinterpret_cast<int4 *>(pb + q) = make_int4(rb[q], rb[q + 1], rb[q + 2], rb[q + 3]);
   }
 
+  // acc = min(acc, As (x) Bs) over the first kend (even) k of the staged
+  // chunk, in this lane's TM x TN cells (rows row0 + 8r, columns
+  // col0 + 32h + c).  One (add, add, min3) triple per two updates, in this
+  // order: on gfx950 v_add_u32 issues from two waves of a SIMD in one
+  // cycle; left to itself the compiler emits all adds of a k pair and then
+  // all mins, so the waves sit in min-only stretches that cannot pair.
+  // Interleaved, the 8192^3 product runs at 68 instead of 50 TOP/s
+  // (measured ceiling of this mix: 76).  The asm is volatile only to keep
+  // the triples in order; it touches registers alone.
+  __device__ static __forceinline__ void compute(const int32_t *As, const int32_t *Bs, int kend,
+                                                 int acc[TM][TN]) {
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int lx = lane & 7, ly = lane >> 3;
+    const int row0 = (wave >> 1) * (8 * TM) + ly;
+    const int col0 = (wave & 1) * (8 * TN) + lx * 4;
+#pragma unroll 1
+    for (int kk = 0; kk < kend; kk += 2) {
+      int a0[TM], a1[TM], b0[TN], b1[TN];
+#pragma unroll
+      for (int r = 0; r < TM; r++) {
+        const int2 v = *reinterpret_cast<const int2 *>(As + (row0 + 8 * r) * MP_AS_LD + kk);
+        a0[r] = v.x;
+        a1[r] = v.y;
+      }
+#pragma unroll
+      for (int h = 0; h < TN / 4; h++) {
+        const int4 u0 = *reinterpret_cast<const int4 *>(Bs + kk * BS_LD + col0 + 32 * h);
+        const int4 u1 = *reinterpret_cast<const int4 *>(Bs + (kk + 1) * BS_LD + col0 + 32 * h);
+        b0[4 * h + 0] = u0.x; b0[4 * h + 1] = u0.y; b0[4 * h + 2] = u0.z; b0[4 * h + 3] = u0.w;
+        b1[4 * h + 0] = u1.x; b1[4 * h + 1] = u1.y; b1[4 * h + 2] = u1.z; b1[4 * h + 3] = u1.w;
+      }
+#pragma unroll
+      for (int r = 0; r < TM; r++)
+#pragma unroll
+        for (int c = 0; c < TN; c++) {
+          int t0, t1;
+          asm volatile(
+              "v_add_u32 %1, %3, %4\n\tv_add_u32 %2, %5, %6\n\tv_min3_i32 %0, %0, %1, %2"
+              : "+v"(acc[r][c]), "=&v"(t0), "=&v"(t1)
+              : "v"(a0[r]), "v"(b0[c]), "v"(a1[r]), "v"(b1[c]));
+        }
+    }
+  }
+
   // Compute the tile.  A points at the tile's first row (k = 0), B at the
   // tile's first column (k = 0), C at the tile's corner; m_valid / n_valid
   // are the in-range rows / columns (<= BM / BN).  `lds` holds LDS_WORDS
@@ -143,40 +188,7 @@ struct MpTile {
       // 116 k, not 128); a k past K reads INF from LDS and changes nothing
       const int krem = K - ch * MP_BK;
       const int kend = krem >= MP_BK ? MP_BK : (krem + 1) & ~1;
-#pragma unroll 1
-      for (int kk = 0; kk < kend; kk += 2) {
-        int a0[TM], a1[TM], b0[TN], b1[TN];
-#pragma unroll
-        for (int r = 0; r < TM; r++) {
-          const int2 v = *reinterpret_cast<const int2 *>(As + (row0 + 8 * r) * MP_AS_LD + kk);
-          a0[r] = v.x;
-          a1[r] = v.y;
-        }
-#pragma unroll
-        for (int h = 0; h < TN / 4; h++) {
-          const int4 u0 = *reinterpret_cast<const int4 *>(Bs + kk * BS_LD + col0 + 32 * h);
-          const int4 u1 = *reinterpret_cast<const int4 *>(Bs + (kk + 1) * BS_LD + col0 + 32 * h);
-          b0[4 * h + 0] = u0.x; b0[4 * h + 1] = u0.y; b0[4 * h + 2] = u0.z; b0[4 * h + 3] = u0.w;
-          b1[4 * h + 0] = u1.x; b1[4 * h + 1] = u1.y; b1[4 * h + 2] = u1.z; b1[4 * h + 3] = u1.w;
-        }
-        // One (add, add, min3) triple per two updates, in this order.  On
-        // gfx950 v_add_u32 issues from two waves of a SIMD in one cycle;
-        // left to itself the compiler emits all adds of a k pair and then
-        // all mins, so the waves sit in min-only stretches that cannot
-        // pair.  Interleaved, the 8192^3 product runs at 68 instead of 50
-        // TOP/s (measured ceiling of this mix: 76).  The asm is volatile
-        // only to keep the triples in order; it touches registers alone.
-#pragma unroll
-        for (int r = 0; r < TM; r++)
-#pragma unroll
-          for (int c = 0; c < TN; c++) {
-            int t0, t1;
-            asm volatile(
-                "v_add_u32 %1, %3, %4\n\tv_add_u32 %2, %5, %6\n\tv_min3_i32 %0, %0, %1, %2"
-                : "+v"(acc[r][c]), "=&v"(t0), "=&v"(t1)
-                : "v"(a0[r]), "v"(b0[c]), "v"(a1[r]), "v"(b1[c]));
-          }
-      }
+      compute(As, Bs, kend, acc);
       __syncthreads();
       if (more) {
         store_chunk(As, Bs, ra, rb);

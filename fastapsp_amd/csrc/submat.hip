@@ -187,6 +187,13 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
     words += ld_sub[p] * n_subs[p];
   }
   if (nsrc > INT32_MAX) return fail(FAP_E_ARG, "subgraph batch: too many sources");
+  // stage 1 through the boundary index when it holds every part as given
+  std::vector<int32_t> hbase(count, -1);
+  bool use_index = ctx->opt_sssp_index && g->bidx && nsrc > 0 && (nsrc + 63) / 64 <= 65535;
+  for (int32_t p = 0; p < count && use_index; p++) {
+    hbase[p] = n_bdys[p] > 0 ? bidx_source_base(g, starts[p], n_subs[p], n_bdys[p]) : 0;
+    use_index = hbase[p] >= 0;
+  }
   FAP_TRY(ensure_ws(ctx, ctx->ws_sub, (size_t)std::max<int64_t>(words, 1) * sizeof(int32_t)));
   std::vector<int32_t *> subs(count);
   for (int32_t p = 0; p < count; p++) subs[p] = (int32_t *)ctx->ws_sub.ptr + off[p];
@@ -206,7 +213,7 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   int32_t **h_rp = (int32_t **)(h + o_rp);
   for (int32_t p = 0, k = 0; p < count; p++)
     for (int32_t i = 0; i < n_bdys[p]; i++, k++) {
-      h_src[k] = g->h_st2ed[starts[p] + i];
+      h_src[k] = use_index ? hbase[p] + i : g->h_st2ed[starts[p] + i];
       h_rp[k] = rowblocks[p] + (int64_t)i * ld_rb;
     }
   memcpy(h + o_jobs, fd.jobs.data(), sizeof(fapk::BatchJob) * fd.jobs.size());
@@ -215,7 +222,9 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   FAP_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
   FAP_TRY(release_host_stage(ctx, s));
   if (sssp_after) FAP_HIP(hipStreamWaitEvent(s, sssp_after, 0));
-  if (nsrc > 0)
+  if (nsrc > 0 && use_index)
+    FAP_TRY(bidx_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, (int32_t *const *)(d + o_rp), s));
+  else if (nsrc > 0)
     FAP_TRY(run_sssp_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, nullptr, 0,
                           (int32_t *const *)(d + o_rp), s));
   if (sssp_done) FAP_HIP(hipEventRecord(sssp_done, s));
@@ -235,14 +244,22 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   return FAP_OK;
 }
 
+// The boundary index is a per-graph cache: built (synchronously) by the
+// first batched solve that may use it.
+static int prepare_index(fap_ctx *ctx, const fap_graph_dev *g) {
+  if (!ctx->opt_sssp_index || g->bidx_state != 0) return FAP_OK;
+  return bidx_build(ctx, const_cast<fap_graph_dev *>(g));
+}
+
 int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                              int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s) {
+  FAP_TRY(prepare_index(ctx, g));
   const int rc = enqueue_subgraph_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb, s);
   const hipError_t e = hipStreamSynchronize(s);
   if (rc != FAP_OK) return rc;
   if (e != hipSuccess) return hip_fail(e, "subgraph batch", __FILE__, __LINE__);
-  return FAP_OK;
+  return bidx_check(g);
 }
 
 // Groups of `group_parts` parts, alternating between ctx's stream and its
@@ -261,7 +278,10 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
   if (!ctx->peer) {
     FAP_TRY(fap_ctx_create(ctx->device, &ctx->peer));
   }
+  FAP_TRY(prepare_index(ctx, g));
   fap_ctx *peer = ctx->peer;
+  peer->opt_sssp_index = ctx->opt_sssp_index;
+  peer->opt_bidx_delta = ctx->opt_bidx_delta;
   peer->opt_fw_lookahead = ctx->opt_fw_lookahead;
   peer->opt_fw_tile = ctx->opt_fw_tile;
   peer->opt_sssp_threads = ctx->opt_sssp_threads;
@@ -287,7 +307,7 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
   if (rc != FAP_OK) return rc;
   if (e1 != hipSuccess) return hip_fail(e1, "subgraph groups", __FILE__, __LINE__);
   if (e2 != hipSuccess) return hip_fail(e2, "subgraph groups", __FILE__, __LINE__);
-  return FAP_OK;
+  return bidx_check(g);
 }
 
 }  // namespace fapi

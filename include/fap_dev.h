@@ -27,6 +27,8 @@
  *   fap_subgraph_solve  <- fapGraph::run            fap/fap.cpp:92-181
  *   fap_batch_*         <- the MPI multi-task driver fap/fap_more_task_mpi.cpp:34-91,
  *                          collapsed into one device work queue on one GPU
+ *   fap_graph_build_index  (stage 1 of batched solves) handle_boundry_path's rows,
+ *                          fap/kernel/batch_sssp.h:254-277, by another exact route
  */
 #ifndef FAP_DEV_H_
 #define FAP_DEV_H_
@@ -83,7 +85,10 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *                           workspace (2) [1]
  *   "sssp_wgs"     0..8     SSSP workgroups per CU [1024 threads per CU]
  *   "sssp_arcs"    0/1/2/4  arcs each lane relaxes per SSSP step, loads issued together [2]
- *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64] */
+ *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64]
+ *   "sssp_index"   0/1      batched part solves compute stage 1 through the graph's
+ *                           boundary index (built by the first such call) [1]
+ *   "bidx_delta"   >= 0     near-far band of the boundary-index SSSP [16 x mean weight] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);
@@ -246,6 +251,22 @@ int fap_pred_rows(fap_ctx *ctx, fap_graph_dev *g, const int32_t *d_dist, int64_t
 int fap_subgraph_pred(fap_ctx *ctx, fap_graph_dev *g, int32_t start, int32_t n_sub,
                       const int32_t *rowblock, int64_t ld_rb, int32_t *predblock, int64_t ld_pred,
                       void *stream);
+
+/* Boundary index of a partitioned graph: for every part, the distances
+ * inside the part from its boundary vertices to its vertices (the closure of
+ * the part's own arcs), plus the graph of boundary vertices (cut arcs and
+ * those in-part distances).  Batched part solves then compute stage 1 -- the
+ * SSSP rows from every boundary vertex, fap/kernel/batch_sssp.h:254-277 --
+ * as an SSSP on that much smaller graph (its distance array in LDS) followed
+ * by one min-plus expansion per part, exactly the same rows.  Built on the
+ * first batched solve (option "sssp_index"); this call builds it at once.
+ * *built = 0 when the graph does not qualify (more than ~36 k boundary
+ * vertices, duplicate arcs, a partition whose parts do not list their
+ * boundary vertices first); stage 1 then runs fap_sssp_multi.  The index is
+ * dropped by fap_graph_set_partition and fap_graph_free.  Synchronous. */
+int fap_graph_build_index(fap_ctx *ctx, fap_graph_dev *g, int32_t *built);
+/* boundary vertices held by the index (0 without one) and its device bytes */
+int fap_graph_index_info(const fap_graph_dev *g, int32_t *boundary, int64_t *bytes);
 
 /* Min-plus product with the path matrix of min_plus_path_CPU_advanced
  * (fap/kernel/minplus.h:51-65): C = min(INF, A (x) B) and, when C_path is

@@ -408,11 +408,20 @@ def test_subgraph_solve_matches_oracle(gpu_ctx, R, K):
     dg.close()
 
 
+@pytest.fixture(params=[1, 0], ids=["index", "sssp"])
+def stage1(gpu_ctx, request):
+    """Stage 1 of batched part solves through the boundary index (default) or
+    the SSSP over the whole graph."""
+    gpu_ctx.set_option("sssp_index", request.param)
+    yield request.param
+    gpu_ctx.set_option("sssp_index", 1)
+
+
 @pytest.mark.parametrize("sssp_config", [(1024, 0, 0), (1024, 0, 0, 2), (512, 0, 8)],
                          indirect=True, ids=_cfg_id)
 @pytest.mark.parametrize("R,K,pick", [(20, 4, None), (33, 7, (7, 2, 5)), (16, 1, None),
                                        (40, 12, (3, 3, 12, 1))])
-def test_subgraph_solve_batch_matches_per_part(gpu_ctx, sssp_config, R, K, pick):
+def test_subgraph_solve_batch_matches_per_part(gpu_ctx, stage1, sssp_config, R, K, pick):
     """All parts (or a subset, duplicates included) in one batched call: every
     row block equals the oracle's per-part pipeline, and a guard column past
     each block's V columns is left alone."""
@@ -430,6 +439,69 @@ def test_subgraph_solve_batch_matches_per_part(gpu_ctx, sssp_config, R, K, pick)
         got = o.get().reshape(-1, ld)
         np.testing.assert_array_equal(got[:, :V], O.run_part(p, ro, col, wt, gid, pc))
         assert (got[:, V:] == -3).all()
+        o.free()
+    dg.close()
+
+
+def _rgg_partitioned(n, K, seed, wl, wh):
+    u, v, w = synth.rgg_graph(n, 6, wl, wh, seed)
+    ro, col, wt = synth.to_csr(n, u, v, w, True)
+    # parts by x coordinate strips of the vertex ids (RGG ids are random)
+    gid = (np.arange(n) * K // n + 1).astype(np.int32)
+    return ro, col, wt, gid, O.precondition(K, gid, ro, col)
+
+
+@pytest.mark.parametrize("n,K,wl,wh", [(900, 6, 1, 255), (1500, 9, 0, 3), (400, 1, 1, 9)])
+def test_boundary_index_rows_equal_sssp_rows(gpu_ctx, n, K, wl, wh):
+    """Directed random geometric graphs (weights 0..3 give equal-distance
+    plateaus; K = 1 has no boundary at all): the batched solve with stage 1
+    through the boundary index equals the one through the whole-graph SSSP
+    and the oracle, part by part."""
+    ro, col, wt, gid, pc = _rgg_partitioned(n, K, 60 + n, wl, wh)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    assert dg.build_index() == (K > 1)           # no boundary vertices: no index needed
+    nb, nbytes = dg.index_info()
+    assert nb == int(pc["bdy_num"][1:].sum()) and (nbytes > 0) == (K > 1)
+    ids = list(range(1, K + 1))
+    got = {}
+    for flag in (1, 0):
+        gpu_ctx.set_option("sssp_index", flag)
+        try:
+            outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), n)) for p in ids]
+            dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in ids],
+                                    [int(pc["ver_num"][p]) for p in ids],
+                                    [int(pc["bdy_num"][p]) for p in ids], [o.ptr for o in outs], n)
+            got[flag] = [o.get() for o in outs]
+            for o in outs:
+                o.free()
+        finally:
+            gpu_ctx.set_option("sssp_index", 1)
+    for i, p in enumerate(ids):
+        np.testing.assert_array_equal(got[1][i], got[0][i])
+        np.testing.assert_array_equal(got[1][i], O.run_part(p, ro, col, wt, gid, pc))
+    dg.close()
+
+
+def test_boundary_index_declines_duplicate_arcs(gpu_ctx):
+    """A graph with a duplicate arc gets no index (its part matrix keeps the
+    last duplicate, as the reference's does); batched solves then run the
+    whole-graph SSSP and still match the oracle."""
+    V, ro, col, wt, gid, pc = partitioned_grid(14, 3, 5)
+    ro2 = ro.copy()
+    col2 = np.insert(col, ro[3], col[ro[3]])      # vertex 3's first arc twice
+    wt2 = np.insert(wt, ro[3], wt[ro[3]] + 7)
+    ro2[4:] += 1
+    dg = gpu_ctx.upload_graph(ro2, col2, wt2)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    assert not dg.build_index()
+    assert dg.index_info() == (0, 0)
+    outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), V)) for p in (1, 2, 3)]
+    dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in (1, 2, 3)],
+                            [int(pc["ver_num"][p]) for p in (1, 2, 3)],
+                            [int(pc["bdy_num"][p]) for p in (1, 2, 3)], [o.ptr for o in outs], V)
+    for p, o in zip((1, 2, 3), outs):
+        np.testing.assert_array_equal(o.get(), O.run_part(p, ro2, col2, wt2, gid, pc))
         o.free()
     dg.close()
 
