@@ -5,6 +5,29 @@
 // reads a tile another task of the launch writes, and every segment runs on
 // the launch's one kernel instance; at the end every matrix equals plain
 // Floyd-Warshall (fap/kernel/floyd.h:38-49, k-i-j order) bit for bit.
+//
+// Second part: a model of ONE persistent launch running the same tasks, in
+// which a workgroup waits (spinning on per-job counters) for its job's
+// previous phase.  The hardware dispatches a grid's workgroups in order of
+// their id as slots free up and never preempts a resident one, so at most
+// R workgroups make progress at a time.  The model replays exactly that
+// (R slots, in-order dispatch, one task step per running workgroup per
+// tick) and reports whether the launch drains:
+//  * tasks dealt statically (workgroup b takes tasks b, b + G, ... of the
+//    phase-ordered list) with G > R workgroups: resident workgroups can all
+//    wait for tasks held by workgroups that are not yet dispatched, which
+//    can only start when a resident one exits -- the launch never ends.
+//    The model must find that state (it is how the removed design hung);
+//  * the same with G <= R (every workgroup resident): drains;
+//  * tasks claimed in list order from a shared counter (a workgroup claims
+//    its next task only when it is running): drains for any G and R, since
+//    the first unfinished task is always claimed, by a running workgroup,
+//    and everything it waits for comes earlier in the list;
+//  * every wait bounded by a spin limit that, when exceeded, raises an
+//    error flag and exits the workgroup: the launch always ends, and the
+//    host sees the flag.
+// Tasks run on the live matrices in the order the model completes them;
+// where the launch drains, every matrix must equal plain Floyd-Warshall.
 // Prints "ok" and exits 0 when everything holds.
 #include <stdint.h>
 #include <stdio.h>
@@ -150,6 +173,167 @@ static bool check_batch(const std::vector<int> &sizes, uint32_t seed) {
   return true;
 }
 
+// ---- persistent-launch model ----------------------------------------------
+
+struct PTask {
+  int job, kind, r, idx;
+  int phase;      // index of this task's launch among its job's launches
+};
+
+struct PList {
+  std::vector<PTask> tasks;                 // phase-ordered (launch by launch)
+  std::vector<std::vector<int>> phase_size;  // [job][phase] task count
+};
+
+static PList persistent_list(const fapi::FwBatchDesc &desc, int count) {
+  PList pl;
+  pl.phase_size.assign(count, {});
+  for (size_t L = 0; L < desc.launch_tasks.size(); L++) {
+    std::vector<int> seen(count, 0);
+    for (int q = desc.seg_off[L]; q < desc.seg_off[L + 1]; q++) {
+      const fapk::BatchSeg &sg = desc.segs[q];
+      const int job = sg.job;
+      if (!seen[job]) {
+        seen[job] = 1;
+        pl.phase_size[job].push_back(0);
+      }
+      const int ph = (int)pl.phase_size[job].size() - 1;
+      for (int idx = 0; idx < sg.count; idx++)
+        pl.tasks.push_back(PTask{job, sg.kind_r & 7, sg.kind_r >> 3, idx, ph});
+      pl.phase_size[job][ph] += sg.count;
+    }
+  }
+  return pl;
+}
+
+static void run_task(std::vector<Mat> &m, const fapi::FwBatchDesc &desc, const PTask &t) {
+  Mat &D = m[t.job];
+  const int nb = desc.jobs[t.job].nb;
+  int I, J, p;
+  if (t.kind == fapk::TASK_DIAG) {
+    close_diag(D, t.r);
+  } else if (t.kind == fapk::TASK_PANEL) {
+    fapk::panel_tile(t.r, nb, t.idx, &I, &J);
+    tile_update(D, D, I, J, t.r, t.r);
+  } else if (t.kind == fapk::TASK_REST2) {
+    fapk::pair_tile(t.r, nb, t.idx, &I, &J);
+    tile_update(D, D, I, J, t.r, t.r + 1);
+  } else {
+    fapk::single_tile(t.kind, t.r, nb, t.idx, &I, &J, &p);
+    tile_update(D, D, I, J, p, p);
+  }
+}
+
+enum class Deal { Static, Claim };
+enum class Outcome { Drained, Stuck, GaveUp };
+
+// G workgroups, at most R resident; spin_limit < 0: unbounded waits.
+static Outcome simulate(const PList &pl, const fapi::FwBatchDesc &desc, std::vector<Mat> &m,
+                        int G, int R, Deal deal, long spin_limit) {
+  const int T = (int)pl.tasks.size();
+  std::vector<std::vector<int>> done(pl.phase_size.size());
+  for (size_t j = 0; j < done.size(); j++) done[j].assign(pl.phase_size[j].size(), 0);
+  auto ready = [&](const PTask &t) {
+    return t.phase == 0 || done[t.job][t.phase - 1] == pl.phase_size[t.job][t.phase - 1];
+  };
+  struct WG { int task = -1; long spins = 0; };
+  std::vector<WG> resident;
+  int next_wg = 0, claim = 0, finished = 0;
+  bool flag = false;
+  // next task of workgroup b after task `cur` (-1: its first); T: none left
+  auto next_task = [&](int b, int cur) {
+    if (deal == Deal::Static) {
+      const long t = cur < 0 ? b : (long)cur + G;
+      return t < T ? (int)t : T;
+    }
+    return claim < T ? claim++ : T;
+  };
+  std::vector<int> wg_id;
+  while (finished < T || !resident.empty()) {
+    while ((int)resident.size() < R && next_wg < G) {  // in-order dispatch into free slots
+      WG w;
+      w.task = next_task(next_wg, -1);
+      resident.push_back(w);
+      wg_id.push_back(next_wg);
+      next_wg++;
+    }
+    bool progress = false;
+    for (size_t i = 0; i < resident.size();) {
+      WG &w = resident[i];
+      if (w.task < T && ready(pl.tasks[w.task])) {
+        run_task(m, desc, pl.tasks[w.task]);
+        done[pl.tasks[w.task].job][pl.tasks[w.task].phase]++;
+        finished++;
+        w.task = next_task(wg_id[i], w.task);
+        w.spins = 0;
+        progress = true;
+      } else if (w.task < T) {
+        if (spin_limit >= 0) {
+          progress = true;  // a bounded wait runs toward its limit
+          if (++w.spins > spin_limit) {
+            flag = true;  // gives up: error flag, workgroup exits
+            w.task = T;
+          }
+        }
+      }
+      if (w.task >= T) {  // exits, freeing its slot
+        resident.erase(resident.begin() + i);
+        wg_id.erase(wg_id.begin() + i);
+        progress = true;
+        continue;
+      }
+      i++;
+    }
+    if (!progress) return Outcome::Stuck;
+    if (resident.empty() && next_wg >= G) break;
+  }
+  if (flag) return Outcome::GaveUp;
+  return finished == T ? Outcome::Drained : Outcome::GaveUp;
+}
+
+static bool check_persistent(const std::vector<int> &sizes, uint32_t seed) {
+  const int count = (int)sizes.size();
+  std::vector<Mat> mats;
+  for (int j = 0; j < count; j++) mats.push_back(random_matrix(sizes[j], seed + 97 * j));
+  std::vector<Mat> want;
+  for (const Mat &m : mats) want.push_back(plain_fw(m));
+  std::vector<int32_t *> ptrs(count, nullptr);
+  std::vector<int32_t> n(sizes.begin(), sizes.end());
+  std::vector<int64_t> ld(sizes.begin(), sizes.end());
+  fapi::FwBatchDesc desc;
+  fapi::build_fw_batch(count, ptrs.data(), n.data(), ld.data(), &desc);
+  const PList pl = persistent_list(desc, count);
+  const int T = (int)pl.tasks.size();
+  auto expect = [&](const char *what, int G, int R, Deal d, long lim, Outcome want_o) {
+    std::vector<Mat> m = mats;
+    const Outcome o = simulate(pl, desc, m, G, R, d, lim);
+    if (o != want_o) {
+      printf("persistent model, %s (G = %d, R = %d, %d tasks): outcome %d, expected %d\n", what,
+             G, R, T, (int)o, (int)want_o);
+      return false;
+    }
+    if (o == Outcome::Drained)
+      for (int j = 0; j < count; j++)
+        if (m[j].a != want[j].a) {
+          printf("persistent model, %s: job %d differs from plain Floyd-Warshall\n", what, j);
+          return false;
+        }
+    return true;
+  };
+  const int R = 4;
+  // the removed design: static deal, more workgroups than slots, unbounded waits
+  if (!expect("static deal, G > R", 4 * R, R, Deal::Static, -1, Outcome::Stuck)) return false;
+  // the same with a bounded wait: every workgroup exits, the flag is raised
+  if (!expect("static deal, G > R, bounded wait", 4 * R, R, Deal::Static, 1000, Outcome::GaveUp))
+    return false;
+  // every workgroup resident
+  if (!expect("static deal, G <= R", R, R, Deal::Static, -1, Outcome::Drained)) return false;
+  // in-order claims: drains whatever the grid
+  for (int G : {1, R, 4 * R, 64 * R})
+    if (!expect("claimed in order", G, R, Deal::Claim, -1, Outcome::Drained)) return false;
+  return true;
+}
+
 int main() {
   const std::vector<std::vector<int>> batches = {
       {1, 5, 63, 64, 65, 127, 128, 129, 200, 256, 300, 385, 449, 577},
@@ -162,6 +346,12 @@ int main() {
     if (!check_batch(b, seed)) return 1;
     seed += 1000;
   }
-  printf("ok: %zu batches\n", batches.size());
+  // persistent-launch model (batches whose jobs have several phases)
+  const std::vector<std::vector<int>> pbatches = {{200, 129, 64}, {385, 70, 5, 128}};
+  for (const auto &b : pbatches) {
+    if (!check_persistent(b, seed)) return 1;
+    seed += 1000;
+  }
+  printf("ok: %zu batches, %zu persistent models\n", batches.size(), pbatches.size());
   return 0;
 }

@@ -187,7 +187,12 @@ def test_batched_closure_schedule_simulated_on_cpu(tmp_path):
     tile maps its kernels use, run as a CPU simulation over ragged batches
     (tests/cpp/batch_schedule_check.cpp): tiles written once per launch, no
     launch reads a tile it writes elsewhere, one kernel instance per launch,
-    and every matrix equal to plain Floyd-Warshall."""
+    and every matrix equal to plain Floyd-Warshall.  The same program models
+    one persistent launch with in-kernel waits on per-job phase counters
+    under in-order dispatch into R resident slots: a static task deal with
+    more workgroups than slots must be found stuck, a bounded wait must end
+    it with the error flag, and in-order task claims must drain for any
+    grid, with every matrix equal to plain Floyd-Warshall."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
