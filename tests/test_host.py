@@ -207,3 +207,97 @@ def test_batched_closure_schedule_simulated_on_cpu(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
+
+
+def _build_metis_tools(tmp_path):
+    import shutil
+    import subprocess
+    gcc, gxx = shutil.which("gcc"), shutil.which("g++")
+    if not gcc or not gxx:
+        pytest.skip("no gcc/g++")
+    lib = str(tmp_path / "libmetis_recorder.so")
+    ref = str(tmp_path / "metis_input_ref")
+    r = subprocess.run([gcc, "-shared", "-fPIC", "-O1", "-o", lib,
+                        os.path.join(ROOT, "tests", "cpp", "metis_recorder.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([gxx, "-O1", "-std=c++17", "-o", ref,
+                        os.path.join(ROOT, "tests", "cpp", "metis_input_ref.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return lib, ref
+
+
+def _reference_metis_input(ref, directed, ro, col):
+    import subprocess
+    text = f"{int(directed)} {len(ro) - 1} {len(col)}\n" + " ".join(map(str, ro)) + "\n" + \
+        " ".join(map(str, col)) + "\n"
+    r = subprocess.run([ref], input=text, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")
+    return [int(x) for x in lines[0].split()], [int(x) for x in lines[1].split()]
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_metis_adapter_hands_metis_what_graph_decomposition_builds(tmp_path, monkeypatch,
+                                                                   directed):
+    """The METIS adapter (partition.cpp) against a recording stand-in for
+    libmetis (tests/cpp/metis_recorder.c): METIS_PartGraphKway receives
+    exactly the xadj / adjncy the reference's Graph_decomposition builds
+    (fap/utils/readIdFile.h:71-112, restated in tests/cpp/metis_input_ref.cpp
+    with the same std::map / std::unordered_set, so neighbour order matches
+    too), ncon 1, nparts K and NULL for every optional argument
+    (readIdFile.h:118-121); the returned part[i] becomes graph id
+    part[i] + 1 (readIdFile.h:127-131)."""
+    lib, ref = _build_metis_tools(tmp_path)
+    rec = str(tmp_path / "record.txt")
+    monkeypatch.setenv("FAP_METIS_LIB", lib)
+    monkeypatch.setenv("FAP_METIS_RECORD", rec)
+    monkeypatch.delenv("FAP_NO_METIS", raising=False)
+    rng = np.random.default_rng(3)
+    V, K = 300, 7
+    if directed:
+        # every vertex on a directed cycle (no isolated vertex), random chords, a self loop
+        u = np.r_[np.arange(V), rng.integers(0, V, 700), 5]
+        v = np.r_[(np.arange(V) + 1) % V, rng.integers(0, V, 700), 5]
+    else:
+        a, b = rng.integers(0, V, 800), rng.integers(0, V, 800)
+        keep = a != b
+        u, v = a[keep], b[keep]     # both arcs of each edge added by to_csr
+    w = rng.integers(1, 50, len(u))
+    ro, col, wt = synth.to_csr(V, u, v, w, directed)
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), K) as g:
+        assert g.used_metis() is True
+        assert g.directed is directed
+        gid = g.getGraphId()
+    lines = open(rec).read().split("\n")
+    nv, ncon, nparts = map(int, lines[0].split())
+    assert (nv, ncon, nparts) == (V, 1, K)
+    assert [int(x) for x in lines[1].split()] == [1, 1, 1, 1, 1, 1]   # all NULL
+    xadj = [int(x) for x in lines[2].split()]
+    adjncy = [int(x) for x in lines[3].split()]
+    want_x, want_a = _reference_metis_input(ref, directed, ro, col)
+    assert xadj == want_x
+    assert adjncy == want_a
+    np.testing.assert_array_equal(gid, (7 * np.arange(V)) % K + 1)
+
+
+def test_metis_adapter_refuses_directed_graph_with_isolated_vertex(tmp_path, monkeypatch):
+    """Documented difference: for a directed graph with a vertex that has no
+    arc (other than a self loop), the reference's symmetrised xadj has fewer
+    than V + 1 entries while it still passes nVertices = V
+    (readIdFile.h:80-112), so METIS reads past the array; the adapter
+    refuses such a graph with an error instead, and METIS is not called."""
+    lib, _ = _build_metis_tools(tmp_path)
+    rec = str(tmp_path / "record.txt")
+    monkeypatch.setenv("FAP_METIS_LIB", lib)
+    monkeypatch.setenv("FAP_METIS_RECORD", rec)
+    monkeypatch.delenv("FAP_NO_METIS", raising=False)
+    V = 10
+    u = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8])     # vertex 9 has no arc
+    v = np.array([1, 2, 3, 4, 5, 6, 7, 8, 0])
+    ro, col, wt = synth.to_csr(V, u, v, np.ones_like(u), True)
+    with pytest.raises(fa.FapError, match="without arcs"):
+        g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), 3)
+        g.close()
+    assert not os.path.exists(rec)
