@@ -301,3 +301,55 @@ def test_metis_adapter_refuses_directed_graph_with_isolated_vertex(tmp_path, mon
         g = fa.Graph.from_csr(ro, col, wt.astype(np.float32), 3)
         g.close()
     assert not os.path.exists(rec)
+
+
+def test_host_code_under_address_and_undefined_sanitizers(tmp_path):
+    """The library's host sources (reader, partitioners, preprocessing,
+    fapGraph and its C ABI, tune cache) built with
+    -fsanitize=address,undefined and driven by tests/cpp/host_sanitize.cpp
+    over valid and malformed inputs (and the METIS adapter through the
+    stand-in libmetis); the kernels' objects are linked as built, no GPU
+    call is made.  Any sanitizer report fails the run."""
+    import glob
+    import shutil
+    import subprocess
+    import concurrent.futures as cf
+    gxx, gcc = shutil.which("g++"), shutil.which("gcc")
+    hip_objs = sorted(glob.glob(os.path.join(ROOT, "fastapsp_amd", "_build", "*.hip.o")))
+    if not gxx or not gcc or not hip_objs:
+        pytest.skip("needs g++/gcc and the built kernel objects (fastapsp_amd/build.py)")
+    csrc = os.path.join(ROOT, "fastapsp_amd", "csrc")
+    flags = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
+             "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+             "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+             "-I" + csrc]
+    srcs = [os.path.join(csrc, f) for f in sorted(os.listdir(csrc)) if f.endswith(".cpp")]
+    srcs.append(os.path.join(ROOT, "tests", "cpp", "host_sanitize.cpp"))
+
+    def compile_one(src):
+        obj = str(tmp_path / (os.path.basename(src) + ".o"))
+        r = subprocess.run([gxx] + flags + ["-c", src, "-o", obj], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        return obj
+
+    with cf.ThreadPoolExecutor(4) as ex:
+        objs = list(ex.map(compile_one, srcs))
+    exe = str(tmp_path / "host_sanitize")
+    r = subprocess.run([gxx, "-fsanitize=address,undefined", "-o", exe] + objs + hip_objs +
+                       ["-L/opt/rocm/lib", "-lamdhip64", "-ldl", "-Wl,-rpath,/opt/rocm/lib"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rec = str(tmp_path / "libmetis_recorder.so")
+    r = subprocess.run([gcc, "-shared", "-fPIC", "-o", rec,
+                        os.path.join(ROOT, "tests", "cpp", "metis_recorder.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    scratch = tmp_path / "scratch"
+    scratch.mkdir()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    env.pop("FAP_NO_METIS", None)
+    r = subprocess.run([exe, str(scratch), rec], capture_output=True, text=True, timeout=600,
+                       env=env)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout + r.stderr)[-4000:]
+    assert "runtime error" not in r.stderr, r.stderr[-4000:]
