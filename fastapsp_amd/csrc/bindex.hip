@@ -12,12 +12,19 @@
 //     the distance from c to c' inside Q alone.  Any path of G between
 //     boundary vertices splits into stretches inside one part, each from a
 //     boundary vertex to a boundary vertex, joined by cut arcs, so
-//     d_H(s, c) = d_G(s, c) for boundary s, c.  H has ~29 k vertices on the
-//     512 x 512 grid cut in 256 parts (G: 262 k), so a source's distance
-//     array (118 KB) lives in LDS: one 1024-thread workgroup per source,
-//     near-far rounds over the improved parts, whose improved boundary
-//     vertices are applied together as rows of L_Q's boundary block (b_Q
-//     consecutive targets, read coalesced, one LDS atomic per target).
+//     d_H(s, c) = d_G(s, c) for boundary s, c.  All boundary vertices of a
+//     part P are solved together, as a block of rows X_P (b_P x all boundary
+//     vertices), by Bellman-Ford over the parts: starting from X_P[., B_P] =
+//     L_P on P's own boundary, each iteration updates the blocks X_P[., B_Q]
+//     of the parts Q that a cut arc enters from a part updated in the
+//     previous one: X_P[., B_Q] = min(X_P[., B_Q], E (x) L_Q|B), where
+//     E[., c] is the best way into Q's boundary vertex c over the cut arcs
+//     c' -> c (min of X_P[., c'] + w) -- a min-plus product with b_P rows,
+//     on the product tile, the cut-arc minimum gathered while the A operand
+//     is staged.  When an iteration changes nothing every arc of H is
+//     satisfied, so X_P is exact (the values are lengths of walks, never too
+//     small).  On the 512x512 grid cut in 256 parts: ~30 iterations, each
+//     (P, Q) block updated ~13 times.
 //  2. expansion.  A path from s to v in part Q enters Q for the last time at
 //     a boundary vertex c of Q (or starts there: s is one), then stays in Q:
 //     d(s, v) = min over c in B_Q of d_H(s, c) + L_Q(c, v).  Per part that
@@ -26,10 +33,11 @@
 //
 // L_Q for every part is computed once per graph (the index): each part's
 // matrix of internal arcs (no boundary rows) closed by the batched closure;
-// its boundary rows are kept, boundary columns for H in st2ed order and all
-// columns in ascending vertex id for the expansion, so that a tile's 64
+// its boundary rows are kept, boundary columns in st2ed order for step 1 and
+// all columns in ascending vertex id for the expansion, so that a tile's 64
 // columns fall on few output cache lines.  The results are the reference's
 // distances exactly (integer, unique).
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -50,255 +58,166 @@ int launch_gather(const fap_graph_dev *g, int32_t start, int32_t n_sub, int32_t 
 
 // One part of the index.  H ids of part q's boundary vertices are
 // hoff .. hoff + b - 1 (hoff a multiple of 4: a few unused ids pad the gaps,
-// so the expansion's A loads are 16-byte aligned).
+// so row blocks of X start 16-byte aligned), in border order: grouped by
+// the neighbouring part each vertex has its first cut arc with, so that the
+// vertices on either side of one border take consecutive ids.  The
+// position of a vertex in that order is its H position (H id - hoff).
 struct BidxPart {
   int32_t start;    // offset of the part in st2ed
   int32_t n;        // vertices
   int32_t b;        // boundary vertices (the part's first b in st2ed)
   int32_t hoff;     // first H id
-  int64_t lbb_off;  // b x b local boundary distances, stride b (H order)
+  int64_t lbb_off;  // b x ldb local boundary distances (H order)
   int64_t lbv_off;  // b x ldv local distances to every vertex, columns by ascending id
+  int32_t ldb;      // b rounded up to a multiple of 4
   int32_t ldv;      // n rounded up to a multiple of 4
-  int32_t pad;
+};
+
+// One 64 x 64 tile of a block update: rows tr*64.. of the block whose first
+// row of X is row0 (rows rows), columns tc*64.. of part q's boundary; the
+// product runs over the kn H positions of q listed at klist[koff..]: the
+// border vertices entered from the parts that changed.  `flag` is its
+// task's changed flag.
+struct BidxTile {
+  int32_t row0, rows, q, tr, tc, flag, koff, kn;
 };
 
 struct fap_bidx {
   int32_t K = 0;
   int32_t nbp = 0;         // H ids, padding included
   int32_t nb = 0;          // boundary vertices
-  int32_t delta = 0;       // near-far band of the boundary SSSP
   int32_t nctiles = 0;     // expansion column tiles (64 columns of one part)
-  std::vector<BidxPart> h_parts;   // host copy, index = part id - 1
+  int32_t delta = 1;       // distance band of the block iterations
+  std::vector<BidxPart> h_parts;          // host copy, index = part id - 1
+  std::vector<std::vector<int2>> nbr;     // {part q2, border} for the cut arcs out of part q
+  std::vector<std::vector<int32_t>> bord; // border: H positions of q2 entered from q, ascending
+  std::vector<int32_t> h_hpos;            // H position of each boundary vertex (local index order)
+  std::vector<int32_t> boff;              // K + 1: each part's first entry in h_hpos
   BidxPart *parts = nullptr;       // device, K
-  int32_t *hpart = nullptr;        // nbp: part index of an H id (-1 for padding)
-  int4 *prec = nullptr;            // K: {hoff, b | x << 11, lbb offset, ext offset}
-  int32_t *ext = nullptr;          // per part: x targets (H id | part << 16), then b x x weights
+  int32_t *in_off = nullptr;       // nbp + 1: cut arcs into each H id
+  int2 *in_arc = nullptr;          // {source H id, weight}
+  int32_t *hloc = nullptr;         // nbp: local (st2ed) index of each H id's vertex
   int32_t *lbb = nullptr;
   int32_t *lbv = nullptr;
   int32_t *colid = nullptr;        // V: vertex ids of each part ascending, at the part's offset
   int2 *ctiles = nullptr;          // {part index, first local column}
-  unsigned int *err = nullptr;     // set when a boundary SSSP exceeds its round bound
   size_t bytes = 0;
 };
 
 namespace fapk {
 
-constexpr int HS_THREADS = 1024;
-constexpr int HS_WAVES = HS_THREADS / 64;
-constexpr int HS_MAX_NB = 32768;  // LDS: 128 KB of distances + 4 KB of improved-vertex bits
-constexpr int HS_MAX_K = 1024;    // LDS: 16 KB of part records + 128 B of improved-part bits
-constexpr int HS_MAX_B = 1024;    // boundary vertices of one part (b < 2^11)
-constexpr int HS_MAX_X = 4096;    // targets of one part's cut arcs (x < 2^13)
-constexpr int HS_ROWS = 8;        // claimed rows applied together
-constexpr int HS_CH = 4;          // 64-column chunks per pass: 32 row loads in flight per lane
-
-// Boundary SSSP, one source per workgroup pass, its distances in LDS, in
-// near-far rounds over parts.  Part q's row for boundary vertex c holds the
-// distances inside q from c to q's b boundary vertices, then the weights of
-// c's cut arcs to the x boundary vertices of other parts that q's cut arcs
-// reach (INF where c has no such arc).  Every improved vertex has its bit
-// set, and then its part's bit.  A round: the part bits are collected into
-// a list (and cleared); the waves take its parts one at a time; for one, a
-// wave claims -- a lane per vertex -- the
-// part's improved vertices whose distance is below the threshold T (their
-// bits cleared; vertices left improved set the part's bit again) and
-// applies them together: lane j computes min over claimed c of d[c] +
-// row_c[j] for columns j, 8 rows x 4 chunks of 64 columns loaded at a time,
-// then lowers each target's distance by one LDS atomic.  Distances are read
-// after the claim, so a later decrease sets the bit anew and the vertex is
-// applied again.  A round that claims nothing raises T to the smallest
-// improved distance + delta (exact: nothing changed during it); a round
-// with no improved vertex ends the source.  Every vertex left unimproved
-// has had its row applied with its final distance, so the result is exact;
-// every distance is the length of a walk, so none is too small.  Rounds are
-// bounded (max_rounds: *err is set and every workgroup stops), so the
-// launch always drains.
-__global__ __launch_bounds__(HS_THREADS) void bidx_sssp_kernel(
-    const int32_t *__restrict__ srcs, int32_t nsrc, int32_t nbp, int32_t K,
-    const int4 *__restrict__ prec, const int32_t *__restrict__ hpart,
-    const int32_t *__restrict__ lbb, const int32_t *__restrict__ ext, int32_t delta,
-    int32_t *__restrict__ X, int64_t ldx, int32_t max_rounds, unsigned int *err) {
-  __shared__ int32_t dist[HS_MAX_NB];
-  __shared__ uint32_t bits[HS_MAX_NB / 32];
-  __shared__ int4 pinfo[HS_MAX_K];  // {hoff, b | x << 11, lbb offset, ext offset}
-  __shared__ uint32_t pbits[HS_MAX_K / 32];
-  __shared__ uint64_t cmask[HS_WAVES][HS_MAX_B / 64];  // a wave's claimed vertices
-  __shared__ int32_t plist[HS_MAX_K];                  // the round's improved parts
-  __shared__ int32_t s_processed, s_nextmin, s_stop, s_np, s_take;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nw = (nbp + 31) >> 5, nkw = (K + 31) >> 5;
-  for (int i = tid; i < K; i += HS_THREADS) pinfo[i] = prec[i];
-  for (int si = blockIdx.x; si < nsrc; si += gridDim.x) {
-    if (tid == 0) s_stop = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    for (int i = tid; i < nbp; i += HS_THREADS) dist[i] = FAP_INF;
-    for (int i = tid; i < nw; i += HS_THREADS) bits[i] = 0u;
-    for (int i = tid; i < nkw; i += HS_THREADS) pbits[i] = 0u;
-    __syncthreads();
-    if (s_stop) return;  // uniform: another workgroup hit the round bound
-    if (tid == 0) {
-      const int h0 = srcs[si], q0 = hpart[h0];
-      dist[h0] = 0;
-      bits[h0 >> 5] = 1u << (h0 & 31);
-      pbits[q0 >> 5] = 1u << (q0 & 31);
-    }
-    int T = delta;
-    int rounds = 0;
-    bool ok = true;
-    for (;;) {
-      // the round's list of improved parts (their bits cleared), taken by
-      // the waves one part at a time
-      if (wave == 0) {
-        int base = 0;
-        for (int w0 = 0; w0 < nkw; w0 += 64) {
-          const int w = w0 + lane;
-          uint32_t word = w < nkw ? pbits[w] : 0u;
-          if (w < nkw) pbits[w] = 0u;
-          const int cnt = __popc(word);
-          int pre = cnt;  // inclusive prefix over the lanes
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(pre, d);
-            if (lane >= d) pre += y;
-          }
-          int slot = base + pre - cnt;
-          while (word) {
-            plist[slot++] = (w << 5) + __ffs(word) - 1;
-            word &= word - 1u;
-          }
-          base += __shfl(pre, 63);
-        }
-        if (lane == 0) {
-          s_np = base;
-          s_take = 0;
-          s_processed = 0;
-          s_nextmin = FAP_INF;
-        }
-      }
-      __syncthreads();
-      int processed = 0, nextmin = FAP_INF;
-      const int np = s_np;
-      for (;;) {
-        int pick = 0;
-        if (lane == 0) pick = atomicAdd(&s_take, 1);
-        pick = __shfl(pick, 0);
-        if (pick >= np) break;
-        const int q = plist[pick];
-        const int4 pi = pinfo[q];
-        const int hoff = pi.x, b = pi.y & 0x7FF, x = pi.y >> 11;
-        const int lbb_off = pi.z, ext_off = pi.w;
-        // claim, a lane per boundary vertex
-        const int nmask = (b + 63) >> 6;
-        int nclaimed = 0;
-        bool anyleft = false;
-        for (int m0 = 0; m0 < nmask; m0++) {
-          const int i = (m0 << 6) + lane, v = hoff + i;
-          bool cl = false, lf = false;
-          if (i < b && ((__hip_atomic_load(&bits[v >> 5], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP) >> (v & 31)) & 1u)) {
-            const int dv = dist[v];
-            cl = dv < T;
-            lf = !cl;
-            if (lf) nextmin = min(nextmin, dv);
-          }
-          const uint64_t m = __ballot(cl);
-          if (lane == 0) cmask[wave][m0] = m;
-          if (cl) atomicAnd(&bits[v >> 5], ~(1u << (v & 31)));
-          nclaimed += __popcll((unsigned long long)m);
-          anyleft = anyleft || __ballot(lf) != 0;
-        }
-        if (anyleft && lane == 0) atomicOr(&pbits[q >> 5], 1u << (q & 31));
-        if (nclaimed == 0) continue;
-        processed++;
-        bool improved = false;
-        const int ncol = b + x;
-        for (int p0 = 0; p0 < ncol; p0 += 64 * HS_CH) {
-          int acc[HS_CH];
-#pragma unroll
-          for (int ch = 0; ch < HS_CH; ch++) acc[ch] = FAP_INF;
-          int mi = 0;
-          uint64_t mcur = cmask[wave][0];
-          for (;;) {
-            int cs[HS_ROWS];
-            int k = 0;
-#pragma unroll
-            for (int g = 0; g < HS_ROWS; g++) {
-              while (mcur == 0 && mi + 1 < nmask) mcur = cmask[wave][++mi];
-              if (mcur) {
-                cs[g] = (mi << 6) + __ffsll((unsigned long long)mcur) - 1;  // row in the part
-                mcur &= mcur - 1;
-                k = g + 1;
-              } else {
-                cs[g] = 0;
-              }
-            }
-            if (k == 0) break;
-            int w[HS_ROWS][HS_CH];
-#pragma unroll
-            for (int g = 0; g < HS_ROWS; g++)
-#pragma unroll
-              for (int ch = 0; ch < HS_CH; ch++) {
-                const int col = p0 + ch * 64 + lane;
-                w[g][ch] = FAP_INF;
-                if (g < k && col < ncol)
-                  w[g][ch] = col < b ? lbb[lbb_off + cs[g] * b + col]
-                                     : ext[ext_off + x + cs[g] * x + (col - b)];
-              }
-#pragma unroll
-            for (int g = 0; g < HS_ROWS; g++)
-              if (g < k) {
-                const int dg = dist[hoff + cs[g]];
-#pragma unroll
-                for (int ch = 0; ch < HS_CH; ch++) acc[ch] = min(acc[ch], dg + w[g][ch]);
-              }
-            if (k < HS_ROWS) break;
-          }
-#pragma unroll
-          for (int ch = 0; ch < HS_CH; ch++) {
-            const int col = p0 + ch * 64 + lane;
-            if (col >= ncol) continue;
-            int u, q2;
-            if (col < b) {
-              u = hoff + col;
-              q2 = q;
-            } else {
-              const int packed = ext[ext_off + col - b];
-              u = packed & 0xFFFF;
-              q2 = packed >> 16;
-            }
-            if (acc[ch] < dist[u]) {
-              const int old = atomicMin(&dist[u], acc[ch]);
-              if (acc[ch] < old) {
-                atomicOr(&bits[u >> 5], 1u << (u & 31));
-                if (q2 == q) improved = true;
-                else atomicOr(&pbits[q2 >> 5], 1u << (q2 & 31));
-              }
-            }
-          }
-        }
-        if (__ballot(improved) && lane == 0) atomicOr(&pbits[q >> 5], 1u << (q & 31));
-      }
-      if (lane == 0 && processed) atomicAdd(&s_processed, processed);
-      if (nextmin < FAP_INF) atomicMin(&s_nextmin, nextmin);  // per lane: lanes read vertices
-      __syncthreads();
-      const int proc = s_processed, nm = s_nextmin;
-      __syncthreads();  // everyone has read both before thread 0 resets them
-      if (proc == 0) {
-        if (nm >= FAP_INF) break;
-        T = nm + delta;
-      }
-      if (++rounds > max_rounds) {
-        ok = false;
-        break;
-      }
-    }
-    if (!ok) {
-      if (tid == 0) atomicOr(err, 1u);
-      return;  // uniform
-    }
-    int32_t *xr = X + (int64_t)si * ldx;
-    for (int i = tid; i < nbp; i += HS_THREADS) xr[i] = dist[i];
-    __syncthreads();  // the next source re-initialises the arrays
+// Start of a block: row r of X (part q, source at H position i) = INF,
+// with L_q(i, .) on q's own boundary columns.
+__global__ void bidx_block_init_kernel(int32_t *__restrict__ X, int64_t ldx, int32_t nbp,
+                                       const int2 *__restrict__ rowq,
+                                       const BidxPart *__restrict__ parts,
+                                       const int32_t *__restrict__ lbb) {
+  const int r = blockIdx.x;
+  const int2 qi = rowq[r];
+  const BidxPart P = parts[qi.x];
+  int32_t *xr = X + (int64_t)r * ldx;
+  for (int c = threadIdx.x; c < nbp; c += blockDim.x) {
+    const int j = c - P.hoff;
+    xr[c] = (j >= 0 && j < P.b) ? lbb[P.lbb_off + (int64_t)qi.y * P.ldb + j] : kInf;
   }
+}
+
+// One tile of a block update X_P[., B_q] = min(X_P[., B_q], E (x) L_q|B):
+// A (64 rows x kn) is E over the listed border positions k of q, staged
+// chunk by chunk straight from the cut arcs into q (E[i][k] = min over arcs
+// c' -> hoff_q + k of X[i][c'] + w); B is the listed rows of L_q's boundary
+// block; C is X, kept where not improved.  flags[T.flag] (INF before the
+// launch) takes the smallest improved value of the task.  Positions left out are entered only from parts whose blocks did
+// not change since this block was last updated with them.  X values read by the gather may be
+// lowered concurrently by other tiles of the launch; either value is the
+// length of a walk, and a lowered one re-flags its block, so the iteration
+// only converges on exact distances.
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void bidx_block_kernel(
+    const BidxTile *__restrict__ tiles, const int32_t *__restrict__ klists, int32_t *X,
+    int64_t ldx, const BidxPart *__restrict__ parts, const int32_t *__restrict__ in_off,
+    const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb, int32_t *flags) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  const BidxTile T = tiles[blockIdx.x];
+  const BidxPart Q = parts[T.q];
+  const int r0 = T.tr * 64, c0 = T.tc * 64;
+  const int m_valid = min(64, T.rows - r0), n_valid = min(64, Q.b - c0), K = T.kn;
+  const int32_t *klist = klists + T.koff;
+  int32_t *Xt = X + (int64_t)(T.row0 + r0) * ldx;
+  const int32_t *B = lbb + Q.lbb_off + c0;
+  const int t = threadIdx.x;
+  const int arow = t >> 2, akq = (t & 3) * 4;  // A staging: row, first k (MpSmall's layout)
+  const int32_t *xa = Xt + (int64_t)arow * ldx;
+  int ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = k0 + akq + j;
+      int v = kInf;
+      if (arow < m_valid && k < K) {
+        const int h = Q.hoff + klist[k];
+        const int e1 = in_off[h + 1];
+        for (int e = in_off[h]; e < e1; e++) {
+          const int2 a = in_arc[e];
+          v = min(v, xa[a.x] + a.y);
+        }
+      }
+      ra[j] = v;
+    }
+    const int kr = k0 + (t >> 4), col = (t & 15) * 4;
+    const int32_t *pb = B + (int64_t)(kr < K ? klist[kr] : 0) * Q.ldb + col;
+    if (kr < K && col + 4 <= n_valid) {
+      const int4 v = *reinterpret_cast<const int4 *>(pb);
+      rb[0] = v.x; rb[1] = v.y; rb[2] = v.z; rb[3] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) rb[j] = (kr < K && col + j < n_valid) ? pb[j] : kInf;
+    }
+  };
+  int acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[i][c] = kInf;
+  int32_t *As = lds;
+  int32_t *Bs = lds + 64 * MP_AS_LD;
+  const int nch = (K + MP_BK - 1) / MP_BK;
+  if (nch > 0) {
+    load(0);
+    MpSmall::store_chunk(As, Bs, ra, rb);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const bool more = ch + 1 < nch;
+    if (more) load((ch + 1) * MP_BK);
+    const int krem = K - ch * MP_BK;
+    MpSmall::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
+    __syncthreads();
+    if (more) {
+      MpSmall::store_chunk(As, Bs, ra, rb);
+      __syncthreads();
+    }
+  }
+  const int wave = t >> 6, lane = t & 63;
+  const int row0 = (wave >> 1) * 32 + (lane >> 3);
+  const int col0 = (wave & 1) * 32 + (lane & 7) * 4;
+  int lowest = kInf;  // smallest improved value
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = row0 + 8 * i;
+    if (row >= m_valid) continue;
+    int32_t *cr = Xt + (int64_t)row * ldx + Q.hoff + c0 + col0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (col0 + c >= n_valid) continue;
+      if (acc[i][c] < cr[c]) {
+        cr[c] = acc[i][c];
+        lowest = min(lowest, acc[i][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off));
+  if (lane == 0 && lowest < kInf) atomicMin(&flags[T.flag], lowest);
 }
 
 // Rows of the expansion: tile (64 sources, 64 columns of part q) =
@@ -363,19 +282,23 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 }
 
 // Index extraction for one part: from its closed local matrix (st2ed
-// order, n x n, stride ld) keep the boundary rows -- columns < b as the
-// part's block of H arcs, every column reordered by ascending vertex id for
-// the expansion.
+// order, n x n, stride ld) keep the boundary rows, in H order: columns of
+// boundary vertices in H order as the part's boundary block (stride ldb),
+// every column by ascending vertex id for the expansion.
 __global__ void bidx_extract_kernel(const int32_t *__restrict__ sub, int64_t ld, BidxPart P,
+                                    const int32_t *__restrict__ hloc,
                                     const int32_t *__restrict__ colid,
                                     const int32_t *__restrict__ ed2st, int32_t *__restrict__ lbb,
                                     int32_t *__restrict__ lbv) {
-  const int i = blockIdx.y;  // boundary row
-  const int32_t *srow = sub + (int64_t)i * ld;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P.ldv; j += gridDim.x * blockDim.x) {
-    if (j < P.b) lbb[P.lbb_off + (int64_t)i * P.b + j] = srow[j];
-    lbv[P.lbv_off + (int64_t)i * P.ldv + j] =
-        j < P.n ? srow[ed2st[colid[P.start + j]] - P.start] : kInf;
+  const int i = blockIdx.y;  // H position of the row
+  const int32_t *srow = sub + (int64_t)hloc[P.hoff + i] * ld;
+  const int jend = max(P.ldv, P.ldb);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jend; j += gridDim.x * blockDim.x) {
+    if (j < P.ldb)
+      lbb[P.lbb_off + (int64_t)i * P.ldb + j] = j < P.b ? srow[hloc[P.hoff + j]] : kInf;
+    if (j < P.ldv)
+      lbv[P.lbv_off + (int64_t)i * P.ldv + j] =
+          j < P.n ? srow[ed2st[colid[P.start + j]] - P.start] : kInf;
   }
 }
 
@@ -387,13 +310,28 @@ namespace {
 
 void free_bidx(fap_bidx *x) {
   if (!x) return;
-  void *bufs[] = {x->parts, x->hpart, x->prec,  x->ext,    x->lbb,
-                  x->lbv,   x->colid, x->ctiles, x->err};
+  void *bufs[] = {x->parts, x->in_off, x->in_arc, x->hloc, x->lbb, x->lbv, x->colid, x->ctiles};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete x;
 }
 
+template <typename T>
+int dev_alloc(int64_t count, T **d, size_t *bytes) {
+  const size_t nb = (size_t)count * sizeof(T);
+  hipError_t e = hipMalloc((void **)d, nb);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *d = nullptr;
+    return fail(FAP_E_NOMEM, "boundary index: device allocation of " + std::to_string(nb) +
+                                 " bytes failed");
+  }
+  *bytes += nb;
+  return FAP_OK;
+}
+
+// Copies h into a new device array (h must outlive the copy: the caller
+// synchronizes s before h goes away).
 template <typename T>
 int upload(const std::vector<T> &h, T **d, size_t *bytes, hipStream_t s) {
   const size_t nb = std::max<size_t>(h.size(), 1) * sizeof(T);
@@ -410,6 +348,8 @@ int upload(const std::vector<T> &h, T **d, size_t *bytes, hipStream_t s) {
   return FAP_OK;
 }
 
+constexpr int32_t kMaxIndexH = 1 << 18;  // boundary vertices (H ids): X rows of 1 MiB at most
+
 }  // namespace
 
 void bidx_release(fap_graph_dev *g) {
@@ -419,11 +359,10 @@ void bidx_release(fap_graph_dev *g) {
 }
 
 // Builds the index of g (partition attached).  Leaves g->bidx null (and
-// returns FAP_OK) when the graph does not qualify: more boundary vertices
-// than the LDS distance array holds, boundary vertices not first in their
-// part's st2ed range, or duplicate arcs (the part matrix keeps the last
-// duplicate, as the reference's does, which need not be the shortest).
-// Synchronous.
+// returns FAP_OK) when the graph does not qualify: more than 2^18 boundary
+// vertices, boundary vertices not first in their part's st2ed range, or
+// duplicate arcs (the part matrix keeps the last duplicate, as the
+// reference's does, which need not be the shortest).  Synchronous.
 int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   if (g->bidx_state) return FAP_OK;
   g->bidx_state = -1;
@@ -483,91 +422,107 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
     }
     P.hoff = (int32_t)hoff;
     hoff += (P.b + 3) & ~3;
+    P.ldb = (P.b + 3) & ~3;
     P.ldv = (P.n + 3) & ~3;
     P.lbb_off = lbb;
-    lbb += (int64_t)P.b * P.b;
+    lbb += (int64_t)P.b * P.ldb;
     P.lbv_off = lbv;
     lbv += (int64_t)P.b * P.ldv;
     pos += P.n;
   }
-  x->nbp = (int32_t)hoff;
-  int32_t bmax = 0;
-  for (const BidxPart &P : x->h_parts) bmax = std::max(bmax, P.b);
-  if (x->nbp > fapk::HS_MAX_NB || x->nbp == 0 || K > fapk::HS_MAX_K || bmax > fapk::HS_MAX_B) {
+  if (hoff == 0 || hoff > kMaxIndexH) {
     delete x;
     return FAP_OK;
   }
+  x->nbp = (int32_t)hoff;
   x->nb = 0;
   for (int32_t q = 1; q <= K; q++) x->nb += bcnt[q];
-  // H ids, cut arcs, columns by ascending id, expansion tiles
-  std::vector<int32_t> hid(V, -1), hpart(x->nbp, -1);
+  {
+    double wsum = 0;
+    for (int64_t e = 0; e < g->E; e++) wsum += ed[e].y;
+    const double mean_w = g->E ? wsum / (double)g->E : 1.0;
+    x->delta = (int32_t)std::min<double>(std::max(1.0, 0.5 * mean_w * std::sqrt((double)V / K)),
+                                         kInf / 4);
+  }
+  // H order: each part's boundary vertices grouped by the neighbouring part
+  // of their first cut arc (either direction), st2ed order within a group
+  std::vector<int32_t> firstnb(V, INT32_MAX);
+  for (int32_t u = 0; u < V; u++)
+    for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
+      const int32_t v = ed[e].x;
+      if (gid[v] == gid[u]) continue;
+      firstnb[u] = std::min(firstnb[u], gid[v]);
+      firstnb[v] = std::min(firstnb[v], gid[u]);
+    }
+  std::vector<int32_t> hid(V, -1), hloc((size_t)x->nbp, 0);
+  x->boff.assign(K + 1, 0);
+  x->h_hpos.assign(x->nb, 0);
   for (int32_t q = 0; q < K; q++) {
     const BidxPart &P = x->h_parts[q];
-    for (int32_t i = 0; i < P.b; i++) {
-      hid[st2ed[P.start + i]] = P.hoff + i;
-      hpart[P.hoff + i] = q;
+    std::vector<int32_t> ord(P.b);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      return firstnb[st2ed[P.start + a]] < firstnb[st2ed[P.start + b]];
+    });
+    x->boff[q + 1] = x->boff[q] + P.b;
+    for (int32_t h = 0; h < P.b; h++) {
+      hid[st2ed[P.start + ord[h]]] = P.hoff + h;
+      hloc[P.hoff + h] = ord[h];
+      x->h_hpos[x->boff[q] + ord[h]] = h;
     }
   }
-  // each part's cut arcs as extra columns of its rows: the distinct targets
-  // (H id | part << 16), then a b x x block of weights (INF: no arc)
-  std::vector<int32_t> ext;
-  std::vector<int4> prec(K);
-  for (int32_t q = 0; q < K; q++) {
-    const BidxPart &P = x->h_parts[q];
-    std::vector<int32_t> tg;  // H ids of the targets
-    for (int32_t i = 0; i < P.b; i++) {
-      const int32_t u = st2ed[P.start + i];
-      for (int32_t e = ro[u]; e < ro[u + 1]; e++)
-        if (gid[ed[e].x] != gid[u]) tg.push_back(hid[ed[e].x]);
-    }
-    std::sort(tg.begin(), tg.end());
-    tg.erase(std::unique(tg.begin(), tg.end()), tg.end());
-    const int32_t nx = (int32_t)tg.size();
-    if (nx > fapk::HS_MAX_X || (int64_t)ext.size() + nx + (int64_t)P.b * nx > INT32_MAX ||
-        P.lbb_off > INT32_MAX) {
-      delete x;
-      return FAP_OK;
-    }
-    prec[q] = make_int4(P.hoff, P.b | (nx << 11), (int32_t)P.lbb_off, (int32_t)ext.size());
-    for (int32_t t : tg) ext.push_back(t | (hpart[t] << 16));
-    const size_t wbase = ext.size();
-    ext.resize(wbase + (size_t)P.b * nx, FAP_INF);
-    for (int32_t i = 0; i < P.b; i++) {
-      const int32_t u = st2ed[P.start + i];
+  // cut arcs into each H id; the borders: for each ordered pair of parts
+  // (q, q2) with a cut arc q -> q2, the H positions of q2 it enters
+  std::vector<int32_t> in_off(x->nbp + 1, 0);
+  std::vector<int2> in_arc;
+  x->nbr.assign(K, {});
+  x->bord.clear();
+  {
+    std::vector<std::vector<int2>> lists(x->nbp);
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> cuts(K);  // part q: {q2, position}
+    for (int32_t u = 0; u < V; u++)
       for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
         const int32_t v = ed[e].x;
         if (gid[v] == gid[u]) continue;
-        const int32_t xi = (int32_t)(std::lower_bound(tg.begin(), tg.end(), hid[v]) - tg.begin());
-        ext[wbase + (size_t)i * nx + xi] = ed[e].y;
+        lists[hid[v]].push_back(make_int2(hid[u], ed[e].y));
+        cuts[gid[u] - 1].push_back({gid[v] - 1, hid[v] - x->h_parts[gid[v] - 1].hoff});
+      }
+    for (int32_t h = 0; h < x->nbp; h++) {
+      in_off[h + 1] = in_off[h] + (int32_t)lists[h].size();
+      in_arc.insert(in_arc.end(), lists[h].begin(), lists[h].end());
+    }
+    for (int32_t q = 0; q < K; q++) {
+      std::vector<std::pair<int32_t, int32_t>> &c = cuts[q];
+      std::sort(c.begin(), c.end());
+      c.erase(std::unique(c.begin(), c.end()), c.end());
+      for (size_t i = 0; i < c.size();) {
+        size_t j = i;
+        std::vector<int32_t> pos;
+        while (j < c.size() && c[j].first == c[i].first) pos.push_back(c[j++].second);
+        x->nbr[q].push_back(make_int2(c[i].first, (int32_t)x->bord.size()));
+        x->bord.push_back(std::move(pos));
+        i = j;
       }
     }
   }
   std::vector<int32_t> colid(st2ed);
   std::vector<int2> ctiles;
-  int64_t wsum = 0;
-  for (int64_t e = 0; e < g->E; e++) wsum += ed[e].y;
   for (int32_t q = 0; q < K; q++) {
     const BidxPart &P = x->h_parts[q];
     std::sort(colid.begin() + P.start, colid.begin() + P.start + P.n);
     for (int32_t j = 0; j < P.n; j += 64) ctiles.push_back(make_int2(q, j));
   }
   x->nctiles = (int32_t)ctiles.size();
-  // near-far band: 64 mean arc weights -- a round applies whole parts, so a
-  // wide band (fewer rounds, more vertices applied again) pays
-  const double mean_w = g->E ? (double)wsum / (double)g->E : 1.0;
-  x->delta = (int32_t)std::min<double>(std::max(1.0, 64.0 * mean_w), (double)(FAP_INF / 4));
   int rc = FAP_OK;
   if (rc == FAP_OK) rc = upload(x->h_parts, &x->parts, &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(hpart, &x->hpart, &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(prec, &x->prec, &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(ext, &x->ext, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(in_off, &x->in_off, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(in_arc, &x->in_arc, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(hloc, &x->hloc, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(colid, &x->colid, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(ctiles, &x->ctiles, &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(std::vector<unsigned int>(1, 0u), &x->err, &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(std::vector<int32_t>((size_t)std::max<int64_t>(lbb, 1)), &x->lbb,
-                                &x->bytes, s);
-  if (rc == FAP_OK) rc = upload(std::vector<int32_t>((size_t)std::max<int64_t>(lbv, 1)), &x->lbv,
-                                &x->bytes, s);
+  // (every word of both is written by the extraction below)
+  if (rc == FAP_OK) rc = dev_alloc(std::max<int64_t>(lbb, 1), &x->lbb, &x->bytes);
+  if (rc == FAP_OK) rc = dev_alloc(std::max<int64_t>(lbv, 1), &x->lbv, &x->bytes);
   // local closures, a few parts at a time (<= 256 MiB of part matrices)
   for (int32_t q0 = 0; q0 < K && rc == FAP_OK;) {
     int32_t q1 = q0;
@@ -599,8 +554,9 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
     for (int32_t i = 0; i < cnt_p && rc == FAP_OK; i++) {
       const BidxPart &P = x->h_parts[q0 + i];
       if (P.b == 0) continue;
-      hipLaunchKernelGGL(fapk::bidx_extract_kernel, dim3((P.ldv + 255) / 256, P.b), dim3(256), 0,
-                         s, subs[i], lds[i], P, x->colid, g->ed2st, x->lbb, x->lbv);
+      hipLaunchKernelGGL(fapk::bidx_extract_kernel,
+                         dim3((std::max(P.ldv, P.ldb) + 255) / 256, P.b), dim3(256), 0, s,
+                         subs[i], lds[i], P, x->hloc, x->colid, g->ed2st, x->lbb, x->lbv);
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) rc = hip_fail(e, "boundary index extract", __FILE__, __LINE__);
     }
@@ -620,49 +576,166 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   return FAP_OK;
 }
 
-// Stage-1 rows of `nsrc` boundary sources through the index: d_hsrc holds
-// their H ids (device), d_rowptr their output rows (device row pointers,
-// V columns each).  Uses ctx->ws_bidx (nsrc x nbp int32).  Asynchronous on s.
-int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_hsrc, int32_t nsrc,
-              int32_t *const *d_rowptr, hipStream_t s) {
+namespace {
+
+// Pinned host and device buffers of the block iterations (tiles, flags),
+// grown on demand.
+int bbf_buffers(fap_ctx *ctx, size_t bytes, char **h, char **d) {
+  if (ctx->host_bbf_bytes < bytes) {
+    if (ctx->host_bbf) (void)hipHostFree(ctx->host_bbf);
+    ctx->host_bbf = nullptr;
+    ctx->host_bbf_bytes = 0;
+    FAP_HIP(hipHostMalloc(&ctx->host_bbf, bytes, hipHostMallocDefault));
+    ctx->host_bbf_bytes = bytes;
+  }
+  FAP_TRY(ensure_ws(ctx, ctx->ws_bbf, bytes));
+  *h = (char *)ctx->host_bbf;
+  *d = (char *)ctx->ws_bbf.ptr;
+  return FAP_OK;
+}
+
+}  // namespace
+
+// Stage-1 rows for the boundary vertices of `count` parts (part indices
+// qidx, X rows from rowoff[p], nsrc rows in all; d_rowptr: device row
+// pointers of the output rows, V columns each, in the same order).  Runs the
+// block iterations from the host (it waits on s each iteration), then
+// enqueues the expansion on s.  Uses ctx->ws_bidx (nsrc x nbp int32).
+int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *qidx,
+              const int32_t *rowoff, int32_t nsrc, int32_t *const *d_rowptr, hipStream_t s) {
   const fap_bidx *x = g->bidx;
   if (!x) return fail(FAP_E_STATE, "no boundary index");
   if (nsrc <= 0) return FAP_OK;
+  const int rows64 = (nsrc + 63) / 64;
+  if (rows64 > 65535) return fail(FAP_E_ARG, "boundary rows: too many sources in one call");
   const int64_t ldx = x->nbp;  // a multiple of 4
   FAP_TRY(ensure_ws(ctx, ctx->ws_bidx, (size_t)nsrc * ldx * sizeof(int32_t)));
   int32_t *X = (int32_t *)ctx->ws_bidx.ptr;
-  const int grid = std::min<int>(nsrc, ctx->num_cus);
-  const int32_t max_rounds = 64 * x->nbp + 4096;
-  hipLaunchKernelGGL(fapk::bidx_sssp_kernel, dim3(grid), dim3(fapk::HS_THREADS), 0, s, d_hsrc,
-                     nsrc, x->nbp, x->K, x->prec, x->hpart, x->lbb, x->ext,
-                     ctx->opt_bidx_delta > 0 ? ctx->opt_bidx_delta : x->delta, X, ldx, max_rounds,
-                     x->err);
+  // blocks start as each part's own boundary distances
+  std::vector<int2> rowq(nsrc);
+  for (int32_t p = 0; p < count; p++) {
+    const int32_t q = qidx[p];
+    for (int32_t i = 0; i < x->h_parts[q].b; i++)
+      rowq[rowoff[p] + i] = make_int2(q, x->h_hpos[x->boff[q] + i]);
+  }
+  char *h = nullptr, *d = nullptr;
+  FAP_TRY(bbf_buffers(ctx, std::max<size_t>(sizeof(int2) * nsrc, 1 << 20), &h, &d));
+  memcpy(h, rowq.data(), sizeof(int2) * nsrc);
+  FAP_HIP(hipMemcpyAsync(d, h, sizeof(int2) * nsrc, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(fapk::bidx_block_init_kernel, dim3(nsrc), dim3(256), 0, s, X, ldx, x->nbp,
+                     (const int2 *)d, x->parts, x->lbb);
   FAP_CHECK_LAUNCH();
-  const int rows = (nsrc + 63) / 64;
-  if (rows > 65535) return fail(FAP_E_ARG, "boundary rows: too many sources in one call");
-  hipLaunchKernelGGL(fapk::bidx_expand_kernel, dim3(x->nctiles, rows), dim3(fapk::MP_THREADS), 0,
+  FAP_HIP(hipStreamSynchronize(s));  // the staging is reused below
+  // Bellman-Ford over the parts, block by block: a block (p, q2) is updated
+  // after a block (p, q) with a cut arc q -> q2 changed, over the border
+  // positions of q2 entered from such parts q.  Changed blocks wait in a
+  // distance band order (delta-stepping over blocks): an iteration
+  // propagates the pending blocks whose smallest changed value lies below
+  // the current threshold, raised by delta when none does, so that most
+  // blocks are propagated once their values are close to final.  Every
+  // change is propagated eventually, so the result does not depend on the
+  // band; values only decrease, so the loop ends.
+  const int32_t K = x->K;
+  const int64_t delta = ctx->opt_bidx_delta > 0 ? ctx->opt_bidx_delta : x->delta;
+  std::vector<int32_t> pend((size_t)count * K, kInf);      // smallest changed value, pending blocks
+  std::vector<std::pair<int32_t, int32_t>> pending, keep, active, tasks;  // (block p, part q)
+  std::vector<std::vector<int32_t>> tbord;                 // borders of each task
+  for (int32_t p = 0; p < count; p++)
+    if (x->h_parts[qidx[p]].b > 0) {
+      pend[(size_t)p * K + qidx[p]] = 0;
+      pending.push_back({p, qidx[p]});
+    }
+  std::vector<int32_t> tix((size_t)count * K, -1);
+  std::vector<BidxTile> tl;
+  std::vector<int32_t> kl;
+  int64_t thr = delta;
+  for (int64_t it = 0; !pending.empty(); it++) {
+    if (it > ((int64_t)1 << 24)) return fail(FAP_E_STATE, "boundary iterations did not settle");
+    int64_t lo = kInf;
+    for (const auto &pq : pending) lo = std::min<int64_t>(lo, pend[(size_t)pq.first * K + pq.second]);
+    if (lo >= thr) thr = (lo / delta + 1) * delta;
+    active.clear();
+    keep.clear();
+    for (const auto &pq : pending) {
+      int32_t &v = pend[(size_t)pq.first * K + pq.second];
+      if (v < thr) {
+        active.push_back(pq);
+        v = kInf;
+      } else {
+        keep.push_back(pq);
+      }
+    }
+    pending.swap(keep);
+    tasks.clear();
+    for (const auto &pq : active)
+      for (const int2 nb : x->nbr[pq.second]) {
+        const int32_t q2 = nb.x;
+        if (x->h_parts[q2].b == 0) continue;
+        int32_t &t = tix[(size_t)pq.first * K + q2];
+        if (t < 0) {
+          t = (int32_t)tasks.size();
+          tasks.push_back({pq.first, q2});
+          if (tbord.size() < tasks.size()) tbord.emplace_back();
+          tbord[t].clear();
+        }
+        tbord[t].push_back(nb.y);
+      }
+    for (const auto &pq : tasks) tix[(size_t)pq.first * K + pq.second] = -1;
+    if (tasks.empty()) continue;
+    tl.clear();
+    kl.clear();
+    for (int32_t j = 0; j < (int32_t)tasks.size(); j++) {
+      const int32_t p = tasks[j].first, q2 = tasks[j].second;
+      const int32_t koff = (int32_t)kl.size();
+      for (int32_t bd : tbord[j]) kl.insert(kl.end(), x->bord[bd].begin(), x->bord[bd].end());
+      if (tbord[j].size() > 1) {
+        std::sort(kl.begin() + koff, kl.end());
+        kl.erase(std::unique(kl.begin() + koff, kl.end()), kl.end());
+      }
+      const int32_t kn = (int32_t)kl.size() - koff;
+      const int32_t rows = x->h_parts[qidx[p]].b, cols = x->h_parts[q2].b;
+      for (int32_t tr = 0; tr * 64 < rows; tr++)
+        for (int32_t tc = 0; tc * 64 < cols; tc++)
+          tl.push_back(BidxTile{rowoff[p], rows, q2, tr, tc, j, koff, kn});
+    }
+    // staging: tiles | border positions | changed flags
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t tbytes = sizeof(BidxTile) * tl.size();
+    const size_t koffb = up(tbytes);
+    const size_t foff = koffb + up(sizeof(int32_t) * kl.size());
+    const size_t fbytes = sizeof(int32_t) * tasks.size();
+    FAP_TRY(bbf_buffers(ctx, foff + fbytes, &h, &d));
+    memcpy(h, tl.data(), tbytes);
+    memcpy(h + koffb, kl.data(), sizeof(int32_t) * kl.size());
+    std::fill((int32_t *)(h + foff), (int32_t *)(h + foff + fbytes), kInf);
+    FAP_HIP(hipMemcpyAsync(d, h, foff + fbytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fapk::bidx_block_kernel, dim3((unsigned)tl.size()), dim3(fapk::MP_THREADS),
+                       0, s, (const BidxTile *)d, (const int32_t *)(d + koffb), X, ldx, x->parts,
+                       x->in_off, x->in_arc, x->lbb, (int32_t *)(d + foff));
+    FAP_CHECK_LAUNCH();
+    FAP_HIP(hipMemcpyAsync(h + foff, d + foff, fbytes, hipMemcpyDeviceToHost, s));
+    FAP_HIP(hipStreamSynchronize(s));
+    const int32_t *fl = (const int32_t *)(h + foff);
+    for (int32_t j = 0; j < (int32_t)tasks.size(); j++) {
+      if (fl[j] >= kInf) continue;
+      int32_t &v = pend[(size_t)tasks[j].first * K + tasks[j].second];
+      if (v >= kInf) pending.push_back(tasks[j]);
+      v = std::min(v, fl[j]);
+    }
+  }
+  hipLaunchKernelGGL(fapk::bidx_expand_kernel, dim3(x->nctiles, rows64), dim3(fapk::MP_THREADS), 0,
                      s, X, ldx, nsrc, d_rowptr, x->ctiles, x->parts, x->lbv, x->colid);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }
 
-int32_t bidx_source_base(const fap_graph_dev *g, int32_t start, int32_t n, int32_t b) {
+int32_t bidx_part(const fap_graph_dev *g, int32_t start, int32_t n, int32_t b) {
   const fap_bidx *x = g->bidx;
   if (!x || start < 0 || start >= g->V || n <= 0) return -1;
   const int32_t q = g->h_graph_id[g->h_st2ed[start]] - 1;
   if (q < 0 || q >= x->K) return -1;
   const BidxPart &P = x->h_parts[q];
-  return (P.start == start && P.n == n && P.b == b) ? P.hoff : -1;
-}
-
-// Raised bound flag of the boundary SSSP (a round count no exact run
-// reaches; reported as an error, the rows are not valid then).
-int bidx_check(const fap_graph_dev *g) {
-  if (!g->bidx) return FAP_OK;
-  unsigned int h = 0;
-  FAP_HIP(hipMemcpy(&h, g->bidx->err, sizeof(h), hipMemcpyDeviceToHost));
-  if (h) return fail(FAP_E_DEVICE, "boundary SSSP exceeded its round bound");
-  return FAP_OK;
+  return (P.start == start && P.n == n && P.b == b) ? q : -1;
 }
 
 }  // namespace fapi

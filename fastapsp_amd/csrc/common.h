@@ -86,7 +86,10 @@ struct fap_ctx {
   fapi::Workspace ws_dense;   // dense-build descriptors
   fapi::Workspace ws_srcs;    // batched-solve sources and row pointers
   fapi::Workspace ws_bidx;    // boundary distances of a batched solve's sources
+  fapi::Workspace ws_bbf;     // boundary iterations: tile lists and changed flags
   uint32_t *host_pinned = nullptr;  // small pinned host buffer for counters
+  void *host_bbf = nullptr;         // pinned staging of the boundary iterations
+  size_t host_bbf_bytes = 0;
   hipEvent_t ev_poll = nullptr;
   // second stream and events for the Floyd-Warshall look-ahead
   hipStream_t aux_stream = nullptr;
@@ -105,7 +108,7 @@ struct fap_ctx {
   int opt_sssp_wgs = 0;          // SSSP workgroups per CU: 1..8; 0 = 1024 threads per CU
   int opt_sssp_arcs = 2;         // arcs per lane per SSSP relaxation step: 1, 2, 4
   int opt_sssp_index = 1;        // batched part solves: stage 1 through the boundary index (1) or SSSP (0)
-  int opt_bidx_delta = 0;        // near-far band of the boundary SSSP; 0 = 16 x mean arc weight
+  int opt_bidx_delta = 0;        // distance band of the boundary iterations; 0 = mean arc weight x sqrt(V / K) / 2
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse
   // once ev_stage (recorded after the last upload from it) has completed
@@ -145,17 +148,14 @@ int release_host_stage(fap_ctx *ctx, hipStream_t s);
 int launch_fill_i32(int32_t *d, int64_t n, int32_t value, hipStream_t s);
 int launch_fill_u32(uint32_t *d, int64_t n, uint32_t value, hipStream_t s);
 // Boundary index (bindex.hip): build (synchronous; leaves g->bidx null when
-// the graph does not qualify), stage-1 rows through it, its error flag,
-// release.
+// the graph does not qualify), stage-1 rows through it (waits on s), release.
 int bidx_build(fap_ctx *ctx, fap_graph_dev *g);
-int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, const int32_t *d_hsrc, int32_t nsrc,
-              int32_t *const *d_rowptr, hipStream_t s);
-int bidx_check(const fap_graph_dev *g);
+int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *qidx,
+              const int32_t *rowoff, int32_t nsrc, int32_t *const *d_rowptr, hipStream_t s);
 void bidx_release(fap_graph_dev *g);
-// H id of boundary vertex i of the part starting at st2ed offset `start`
-// with n vertices and b boundary ones, or -1 when the index does not hold
-// that part as given.
-int32_t bidx_source_base(const fap_graph_dev *g, int32_t start, int32_t n, int32_t b);
+// Index of the part starting at st2ed offset `start` with n vertices and b
+// boundary ones, or -1 when the index does not hold that part as given.
+int32_t bidx_part(const fap_graph_dev *g, int32_t start, int32_t n, int32_t b);
 }  // namespace fapi
 
 namespace fapk {

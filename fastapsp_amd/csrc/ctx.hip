@@ -202,11 +202,12 @@ int fap_ctx_destroy(fap_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   Workspace *all[] = {&ctx->ws_sub, &ctx->ws_queue, &ctx->ws_stamp, &ctx->ws_small,
                       &ctx->ws_misc, &ctx->ws_batch, &ctx->ws_dense, &ctx->ws_srcs,
-                      &ctx->ws_bidx};
+                      &ctx->ws_bidx, &ctx->ws_bbf};
   for (Workspace *w : all)
     if (w->ptr) (void)hipFree(w->ptr);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+  if (ctx->host_bbf) (void)hipHostFree(ctx->host_bbf);
   if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
   if (ctx->ev_sssp) (void)hipEventDestroy(ctx->ev_sssp);
   if (ctx->ev_poll) (void)hipEventDestroy(ctx->ev_poll);
@@ -256,7 +257,7 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
     if (value < 0 || value > 1) return fail(FAP_E_ARG, "sssp_index must be 0 or 1");
     ctx->opt_sssp_index = (int)value;
   } else if (k == "bidx_delta") {
-    if (value < 0 || value > INT32_MAX / 4) return fail(FAP_E_ARG, "bidx_delta out of range");
+    if (value < 0 || value > FAP_INF / 4) return fail(FAP_E_ARG, "bidx_delta out of range");
     ctx->opt_bidx_delta = (int)value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 64) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 64");

@@ -188,11 +188,12 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   }
   if (nsrc > INT32_MAX) return fail(FAP_E_ARG, "subgraph batch: too many sources");
   // stage 1 through the boundary index when it holds every part as given
-  std::vector<int32_t> hbase(count, -1);
+  std::vector<int32_t> qidx(count, -1), rowoff(count, 0);
   bool use_index = ctx->opt_sssp_index && g->bidx && nsrc > 0 && (nsrc + 63) / 64 <= 65535;
-  for (int32_t p = 0; p < count && use_index; p++) {
-    hbase[p] = n_bdys[p] > 0 ? bidx_source_base(g, starts[p], n_subs[p], n_bdys[p]) : 0;
-    use_index = hbase[p] >= 0;
+  for (int32_t p = 0, r = 0; p < count && use_index; r += n_bdys[p], p++) {
+    qidx[p] = bidx_part(g, starts[p], n_subs[p], n_bdys[p]);
+    rowoff[p] = r;
+    use_index = qidx[p] >= 0;
   }
   FAP_TRY(ensure_ws(ctx, ctx->ws_sub, (size_t)std::max<int64_t>(words, 1) * sizeof(int32_t)));
   std::vector<int32_t *> subs(count);
@@ -213,7 +214,7 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   int32_t **h_rp = (int32_t **)(h + o_rp);
   for (int32_t p = 0, k = 0; p < count; p++)
     for (int32_t i = 0; i < n_bdys[p]; i++, k++) {
-      h_src[k] = use_index ? hbase[p] + i : g->h_st2ed[starts[p] + i];
+      h_src[k] = g->h_st2ed[starts[p] + i];
       h_rp[k] = rowblocks[p] + (int64_t)i * ld_rb;
     }
   memcpy(h + o_jobs, fd.jobs.data(), sizeof(fapk::BatchJob) * fd.jobs.size());
@@ -223,7 +224,8 @@ int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
   FAP_TRY(release_host_stage(ctx, s));
   if (sssp_after) FAP_HIP(hipStreamWaitEvent(s, sssp_after, 0));
   if (nsrc > 0 && use_index)
-    FAP_TRY(bidx_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, (int32_t *const *)(d + o_rp), s));
+    FAP_TRY(bidx_rows(ctx, g, count, qidx.data(), rowoff.data(), (int32_t)nsrc,
+                      (int32_t *const *)(d + o_rp), s));
   else if (nsrc > 0)
     FAP_TRY(run_sssp_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, nullptr, 0,
                           (int32_t *const *)(d + o_rp), s));
@@ -259,7 +261,7 @@ int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
   const hipError_t e = hipStreamSynchronize(s);
   if (rc != FAP_OK) return rc;
   if (e != hipSuccess) return hip_fail(e, "subgraph batch", __FILE__, __LINE__);
-  return bidx_check(g);
+  return FAP_OK;
 }
 
 // Groups of `group_parts` parts, alternating between ctx's stream and its
@@ -307,7 +309,7 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
   if (rc != FAP_OK) return rc;
   if (e1 != hipSuccess) return hip_fail(e1, "subgraph groups", __FILE__, __LINE__);
   if (e2 != hipSuccess) return hip_fail(e2, "subgraph groups", __FILE__, __LINE__);
-  return bidx_check(g);
+  return FAP_OK;
 }
 
 }  // namespace fapi

@@ -88,7 +88,8 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64]
  *   "sssp_index"   0/1      batched part solves compute stage 1 through the graph's
  *                           boundary index (built by the first such call) [1]
- *   "bidx_delta"   >= 0     near-far band of the boundary-index SSSP [16 x mean weight] */
+ *   "bidx_delta"   >= 0     distance band of the boundary index's block iterations; any
+ *                           value gives the same rows [mean arc weight x sqrt(V / K) / 2] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);

@@ -483,6 +483,46 @@ def test_boundary_index_rows_equal_sssp_rows(gpu_ctx, n, K, wl, wh):
     dg.close()
 
 
+def test_boundary_index_survives_staging_growth_between_calls():
+    """Regression: a batched solve whose descriptor staging outgrows the
+    previous call's (40 parts, ~12,000 sources, after a one-part call) must
+    leave the boundary iterations' own pinned buffer alone.  A fresh context
+    runs one part, then all 40, then one again; every part equals the
+    whole-graph SSSP path (checksums), the two end parts the oracle."""
+    V, ro, col, wt, gid, pc = partitioned_grid(160, 40, 21)
+    K = 40
+    ctx = fa.Context(0)
+    try:
+        dg = ctx.upload_graph(ro, col, wt)
+        dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+        assert dg.build_index()
+        print(f"[bidx staging] V={V} K={K} sources={int(pc['bdy_num'][1:].sum())}", flush=True)
+        outs = {p: ctx.alloc((int(pc["ver_num"][p]), V)) for p in range(1, K + 1)}
+
+        def solve(ids):
+            dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in ids],
+                                    [int(pc["ver_num"][p]) for p in ids],
+                                    [int(pc["bdy_num"][p]) for p in ids],
+                                    [outs[p].ptr for p in ids], V)
+            return {p: ctx.checksum(outs[p].ptr, int(pc["ver_num"][p]) * V) for p in ids}
+
+        first = solve([3])
+        every = solve(list(range(1, K + 1)))
+        last = solve([5])
+        for p in (1, 40):
+            np.testing.assert_array_equal(outs[p].get(), O.run_part(p, ro, col, wt, gid, pc),
+                                          err_msg=f"part {p}")
+        ctx.set_option("sssp_index", 0)
+        plain = solve(list(range(1, K + 1)))
+        assert every == plain
+        assert first[3] == plain[3] and last[5] == plain[5]
+        for o in outs.values():
+            o.free()
+        dg.close()
+    finally:
+        ctx.close()
+
+
 def test_boundary_index_declines_duplicate_arcs(gpu_ctx):
     """A graph with a duplicate arc gets no index (its part matrix keeps the
     last duplicate, as the reference's does); batched solves then run the
