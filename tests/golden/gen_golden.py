@@ -4,7 +4,8 @@ Expected outputs come from scipy.sparse.csgraph (Dijkstra) and numpy brute
 force -- neither the reference nor this repository's code.  Inputs come from
 fastapsp_amd.synth (seeded, portable).  The reference holds no golden vectors
 of its own for this path; its one fixture graph (luxembourg_osm.mtx, read
-only where the FastAPSP checkout is present) contributes two SSSP rows.
+only where the FastAPSP checkout is present) contributes two SSSP rows and
+is kept itself as its edge lines (luxembourg_osm.npz).
 
     python tests/golden/gen_golden.py
 """
@@ -71,6 +72,11 @@ def main():
         rows = apsp_scipy(V, ro, col, wt, directed=True, indices=src)
         np.savez_compressed(os.path.join(HERE, "luxembourg_rows.npz"), sources=src, rows=rows,
                             V=V, lines=m_)
+        # 6. the fixture graph itself (its header and edge lines, 1-based as in
+        # the file), so that the reference's documented run can be replayed
+        # where the reference checkout is absent
+        np.savez_compressed(os.path.join(HERE, "luxembourg_osm.npz"), V=V, lines=m_,
+                            edges=(e + 1).astype(np.int32))
     print("golden vectors written to", HERE)
 
 

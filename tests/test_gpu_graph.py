@@ -149,3 +149,44 @@ def test_batched_and_plain_solve_agree():
             for pid in (1, 4, 9):
                 np.testing.assert_array_equal(g.getSubGraphDistanceInt32(pid),
                                               want[vo[pid]:vo[pid] + vn[pid]])
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reference_documented_run_luxembourg_k128(tmp_path, monkeypatch):
+    """The reference's documented run, `singleNodeExample -f luxembourg_osm
+    -k 128 -direct false -weight false` (README.md:57, app/README.md:3,
+    app/singleNodeExample.cpp:40-68), on its own fixture graph, replayed
+    byte for byte from tests/golden/luxembourg_osm.npz into the reference's
+    directory layout: fapGraph from the file, preCondition, split, parts 1-3
+    solved with paths, the example's check_result on the current part (3),
+    sampled rows of every part equal to Dijkstra, and part 3's predecessors
+    valid under the reference's path semantics."""
+    z = np.load(os.path.join(GOLDEN, "luxembourg_osm.npz"))
+    V, lines, edges = int(z["V"]), int(z["lines"]), z["edges"]
+    d = tmp_path / "graph" / "unweight-undirected"
+    d.mkdir(parents=True)
+    with open(d / "luxembourg_osm.mtx", "w") as f:
+        f.write(f"{V} {V} {lines}\n")
+        f.write("".join(f"{a} {b}\n" for a, b in edges))
+    monkeypatch.chdir(tmp_path)
+    with fa.Graph.from_file("luxembourg_osm", False, False, 128) as g:
+        assert g.num_vertexs == V
+        ro, col, wt = g.csr()
+        g.preCondition()
+        assert g.isSplit()
+        rng = np.random.default_rng(128)
+        for pid in (1, 2, 3):            # each solve replaces the previous part's rows
+            g.solveSubGraph(pid, True)
+            idx = g.getSubGraphIndex(pid)
+            n = len(idx)
+            rs = np.unique(np.r_[0, n - 1, rng.integers(0, n, 4)]).astype(np.int32)
+            want = O.batched_sssp(idx[rs], ro, col, wt)
+            for k, r in enumerate(rs):
+                np.testing.assert_array_equal(g.getSubGraphRowsInt32(pid, int(r), 1)[0], want[k],
+                                              err_msg=f"part {pid} row {r}")
+        assert g.getCurrentSubGraphId() == 3
+        assert g.check_result(3)
+        D, P = g.getSubGraphDistanceInt32(3), g.getSubGraphPath(3)
+        assert O.check_pred(D, P, g.getSubGraphIndex(3), ro, col, wt) == 0

@@ -110,6 +110,18 @@ def test_dijkstra_on_reference_fixture_graph():
     np.testing.assert_array_equal(got, d["rows"])
 
 
+def test_fixture_graph_copy_gives_the_golden_rows():
+    """tests/golden/luxembourg_osm.npz (the reference's fixture graph kept as
+    its edge lines) reproduces the two scipy Dijkstra rows taken from the
+    original file, so the copy can stand in for it where the reference
+    checkout is absent."""
+    z, d = golden("luxembourg_osm.npz"), golden("luxembourg_rows.npz")
+    V, e = int(z["V"]), z["edges"].astype(np.int64) - 1
+    assert int(z["lines"]) == len(e) == 119666 and V == int(d["V"]) == 114599
+    ro, col, w = synth.to_csr(V, e[:, 0], e[:, 1], np.ones(len(e), np.int64), False)
+    np.testing.assert_array_equal(O.batched_sssp(d["sources"], ro, col, w), d["rows"])
+
+
 # ---------- closed forms ----------
 
 def test_unit_grid_is_manhattan():
