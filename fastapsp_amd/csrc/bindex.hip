@@ -75,12 +75,22 @@ struct BidxPart {
 
 // One 64 x 64 tile of a block update: rows tr*64.. of the block whose first
 // row of X is row0 (rows rows), columns tc*64.. of part q's boundary; the
-// product runs over the kn H positions of q listed at klist[koff..]: the
-// border vertices entered from the parts that changed.  `flag` is its
-// task's changed flag.
+// product runs over the border positions of q entered from the parts
+// whose bits are set in `slots` (bit s: q's s-th in-neighbour part; bit 31
+// stands for every in-neighbour from the 31st on).  `task` indexes the
+// task's smallest-improvement word.
 struct BidxTile {
-  int32_t row0, rows, q, tr, tc, flag, koff, kn;
+  int32_t row0, rows, q, tr, tc, task;
+  uint32_t slots;
+  int32_t pad;
 };
+
+// Control words of the block iterations (device).
+enum { BC_THR = 0, BC_NTASKS = 1, BC_NTILES = 2, BC_DONE = 3, BC_WORDS = 16 };
+
+// Longest border-position list a tile may take (the sum of a part's
+// in-border lengths); a graph with a longer one gets no index.
+constexpr int kBidxKlMax = 2048;
 
 struct fap_bidx {
   int32_t K = 0;
@@ -89,14 +99,23 @@ struct fap_bidx {
   int32_t nctiles = 0;     // expansion column tiles (64 columns of one part)
   int32_t delta = 1;       // distance band of the block iterations
   std::vector<BidxPart> h_parts;          // host copy, index = part id - 1
-  std::vector<std::vector<int2>> nbr;     // {part q2, border} for the cut arcs out of part q
-  std::vector<std::vector<int32_t>> bord; // border: H positions of q2 entered from q, ascending
   std::vector<int32_t> h_hpos;            // H position of each boundary vertex (local index order)
   std::vector<int32_t> boff;              // K + 1: each part's first entry in h_hpos
+  int32_t max_ct = 0;                     // sum over parts of their 64-column tiles
   BidxPart *parts = nullptr;       // device, K
   int32_t *in_off = nullptr;       // nbp + 1: cut arcs into each H id
   int2 *in_arc = nullptr;          // {source H id, weight}
   int32_t *hloc = nullptr;         // nbp: local (st2ed) index of each H id's vertex
+  // borders: part q's in-neighbour slots inb_off[q] .. inb_off[q + 1] - 1;
+  // slot e's H positions of q (entered by cut arcs from that neighbour,
+  // ascending) at inb_pos[inb_pos_off[e] .. inb_pos_off[e + 1] - 1]; part
+  // q's out-neighbours outb[outb_off[q] ..] = {part q2, q's slot index among
+  // q2's in-neighbours}
+  int32_t *inb_off = nullptr;      // K + 1
+  int32_t *inb_pos_off = nullptr;  // slots + 1
+  int32_t *inb_pos = nullptr;
+  int32_t *outb_off = nullptr;     // K + 1
+  int2 *outb = nullptr;
   int32_t *lbb = nullptr;
   int32_t *lbv = nullptr;
   int32_t *colid = nullptr;        // V: vertex ids of each part ascending, at the part's offset
@@ -122,102 +141,207 @@ __global__ void bidx_block_init_kernel(int32_t *__restrict__ X, int64_t ldx, int
   }
 }
 
-// One tile of a block update X_P[., B_q] = min(X_P[., B_q], E (x) L_q|B):
-// A (64 rows x kn) is E over the listed border positions k of q, staged
-// chunk by chunk straight from the cut arcs into q (E[i][k] = min over arcs
-// c' -> hoff_q + k of X[i][c'] + w); B is the listed rows of L_q's boundary
-// block; C is X, kept where not improved.  flags[T.flag] (INF before the
-// launch) takes the smallest improved value of the task.  Positions left out are entered only from parts whose blocks did
-// not change since this block was last updated with them.  X values read by the gather may be
-// lowered concurrently by other tiles of the launch; either value is the
-// length of a walk, and a lowered one re-flags its block, so the iteration
-// only converges on exact distances.
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void bidx_block_kernel(
-    const BidxTile *__restrict__ tiles, const int32_t *__restrict__ klists, int32_t *X,
-    int64_t ldx, const BidxPart *__restrict__ parts, const int32_t *__restrict__ in_off,
-    const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb, int32_t *flags) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
-  const BidxTile T = tiles[blockIdx.x];
-  const BidxPart Q = parts[T.q];
-  const int r0 = T.tr * 64, c0 = T.tc * 64;
-  const int m_valid = min(64, T.rows - r0), n_valid = min(64, Q.b - c0), K = T.kn;
-  const int32_t *klist = klists + T.koff;
-  int32_t *Xt = X + (int64_t)(T.row0 + r0) * ldx;
-  const int32_t *B = lbb + Q.lbb_off + c0;
-  const int t = threadIdx.x;
-  const int arow = t >> 2, akq = (t & 3) * 4;  // A staging: row, first k (MpSmall's layout)
-  const int32_t *xa = Xt + (int64_t)arow * ldx;
-  int ra[4], rb[4];
-  auto load = [&](int k0) {
+// Scheduling step of the block iterations (one workgroup).  Folds the
+// previous launch's per-task smallest improvements into the pending blocks
+// (pend[p * K + q], INF = not pending), raises the band threshold past the
+// smallest pending value when none lies below it, takes every pending
+// block below the threshold (clearing it) and marks, for each part q2 a
+// cut arc from its part q enters, the slot of q among q2's in-neighbours in
+// mask[p * K + q2]; then lists the marked blocks as tasks and their 64 x 64
+// tiles.  With nothing pending it sets the done word and lists nothing, so
+// launches enqueued after convergence are no-ops.
+__global__ __launch_bounds__(1024) void bidx_schedule_kernel(
+    int32_t *ctrl, int32_t *pend, uint32_t *mask, int32_t *tasks, int32_t *tmin,
+    BidxTile *tiles, const int2 *__restrict__ prow, int32_t count, int32_t K, int32_t delta,
+    const BidxPart *__restrict__ parts, const int32_t *__restrict__ outb_off,
+    const int2 *__restrict__ outb) {
+  __shared__ int s_red[16];
+  __shared__ int s_thr, s_nt, s_ntl;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nblk = count * K;
+  const int prev = ctrl[BC_NTASKS];
+  for (int t = tid; t < prev; t += nthr) {
+    const int f = tmin[t];
+    if (f < kInf) atomicMin(&pend[tasks[t]], f);
+  }
+  __threadfence();
+  __syncthreads();
+  int lo = kInf;
+  for (int i = tid; i < nblk; i += nthr) lo = min(lo, pend[i]);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int k = k0 + akq + j;
-      int v = kInf;
-      if (arow < m_valid && k < K) {
-        const int h = Q.hoff + klist[k];
-        const int e1 = in_off[h + 1];
-        for (int e = in_off[h]; e < e1; e++) {
-          const int2 a = in_arc[e];
-          v = min(v, xa[a.x] + a.y);
-        }
-      }
-      ra[j] = v;
-    }
-    const int kr = k0 + (t >> 4), col = (t & 15) * 4;
-    const int32_t *pb = B + (int64_t)(kr < K ? klist[kr] : 0) * Q.ldb + col;
-    if (kr < K && col + 4 <= n_valid) {
-      const int4 v = *reinterpret_cast<const int4 *>(pb);
-      rb[0] = v.x; rb[1] = v.y; rb[2] = v.z; rb[3] = v.w;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; j++) rb[j] = (kr < K && col + j < n_valid) ? pb[j] : kInf;
-    }
-  };
-  int acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) acc[i][c] = kInf;
-  int32_t *As = lds;
-  int32_t *Bs = lds + 64 * MP_AS_LD;
-  const int nch = (K + MP_BK - 1) / MP_BK;
-  if (nch > 0) {
-    load(0);
-    MpSmall::store_chunk(As, Bs, ra, rb);
+  for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off));
+  if ((tid & 63) == 0) s_red[tid >> 6] = lo;
+  __syncthreads();
+  if (tid == 0) {
+    int m = kInf;
+    for (int w = 0; w < (nthr + 63) / 64; w++) m = min(m, s_red[w]);
+    int thr = ctrl[BC_THR];
+    if (m < kInf && m >= thr) thr = (int)(((int64_t)m / delta + 1) * delta);
+    s_thr = m < kInf ? thr : -1;
+    ctrl[BC_THR] = thr;
+    ctrl[BC_DONE] = m < kInf ? 0 : 1;
+    s_nt = 0;
+    s_ntl = 0;
   }
   __syncthreads();
-  for (int ch = 0; ch < nch; ch++) {
-    const bool more = ch + 1 < nch;
-    if (more) load((ch + 1) * MP_BK);
-    const int krem = K - ch * MP_BK;
-    MpSmall::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
-    __syncthreads();
-    if (more) {
-      MpSmall::store_chunk(As, Bs, ra, rb);
-      __syncthreads();
+  const int thr = s_thr;
+  if (thr < 0) {
+    if (tid == 0) {
+      ctrl[BC_NTASKS] = 0;
+      ctrl[BC_NTILES] = 0;
+    }
+    return;
+  }
+  for (int i = tid; i < nblk; i += nthr) {
+    if (pend[i] >= thr) continue;
+    pend[i] = kInf;
+    const int p = i / K, q = i - p * K;
+    for (int e = outb_off[q]; e < outb_off[q + 1]; e++) {
+      const int2 o = outb[e];
+      if (parts[o.x].b > 0) atomicOr(&mask[p * K + o.x], 1u << min(o.y, 31));
     }
   }
+  __threadfence();
+  __syncthreads();
+  for (int i = tid; i < nblk; i += nthr) {
+    const uint32_t m = mask[i];
+    if (!m) continue;
+    mask[i] = 0;
+    const int p = i / K, q2 = i - p * K;
+    const int t = atomicAdd(&s_nt, 1);
+    tasks[t] = i;
+    tmin[t] = kInf;
+    const int2 pr = prow[p];
+    const int rt = (pr.y + 63) >> 6, ct = (parts[q2].b + 63) >> 6;
+    const int base = atomicAdd(&s_ntl, rt * ct);
+    for (int tr = 0; tr < rt; tr++)
+      for (int tc = 0; tc < ct; tc++)
+        tiles[base + tr * ct + tc] = BidxTile{pr.x, pr.y, q2, tr, tc, t, m, 0};
+  }
+  __syncthreads();
+  if (tid == 0) {
+    ctrl[BC_NTASKS] = s_nt;
+    ctrl[BC_NTILES] = s_ntl;
+  }
+}
+
+// Block updates of one iteration, the tiles listed by the scheduling step
+// dealt grid-stride.  Tile: X_P[., B_q] = min(X_P[., B_q], E (x) L_q[k, .])
+// over the border positions k of q entered from the marked in-neighbours
+// (collected in LDS first); A (64 rows x k) is E, staged chunk by chunk
+// straight from the cut arcs into q (E[i][k] = min over arcs c' -> hoff_q +
+// k of X[i][c'] + w); B is the listed rows of L_q's boundary block; C is X,
+// kept where not improved.  The task's word in tmin takes the smallest
+// improved value.  Positions left out are entered only from parts whose
+// blocks did not change since this block was last updated with them.  X
+// values read by the gather may be lowered concurrently by other tiles;
+// either value is the length of a walk, and a lowered one is pending again,
+// so the iteration only converges on exact distances.
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void bidx_block_kernel(
+    const int32_t *__restrict__ ctrl, const BidxTile *__restrict__ tiles, int32_t *X, int64_t ldx,
+    const BidxPart *__restrict__ parts, const int32_t *__restrict__ in_off,
+    const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb,
+    const int32_t *__restrict__ inb_off, const int32_t *__restrict__ inb_pos_off,
+    const int32_t *__restrict__ inb_pos, int32_t *tmin) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  __shared__ int32_t klist[kBidxKlMax];
+  const int ntiles = ctrl[BC_NTILES];
+  const int t = threadIdx.x;
+  const int arow = t >> 2, akq = (t & 3) * 4;  // A staging: row, first k (MpSmall's layout)
   const int wave = t >> 6, lane = t & 63;
   const int row0 = (wave >> 1) * 32 + (lane >> 3);
   const int col0 = (wave & 1) * 32 + (lane & 7) * 4;
-  int lowest = kInf;  // smallest improved value
+  for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    const BidxTile T = tiles[ti];
+    const BidxPart Q = parts[T.q];
+    // border positions of the marked in-neighbours (a corner vertex may
+    // appear twice: min-plus does not mind)
+    int kn = 0;
+    const int s0 = inb_off[T.q], s1 = inb_off[T.q + 1];
+    for (int sl = s0; sl < s1; sl++) {
+      const int si = sl - s0;
+      if (!((T.slots >> min(si, 31)) & 1u)) continue;
+      const int p0 = inb_pos_off[sl], len = inb_pos_off[sl + 1] - p0;
+      for (int i = t; i < len; i += MP_THREADS) klist[kn + i] = inb_pos[p0 + i];
+      kn += len;
+    }
+    const int K = kn;
+    __syncthreads();
+    const int r0 = T.tr * 64, c0 = T.tc * 64;
+    const int m_valid = min(64, T.rows - r0), n_valid = min(64, Q.b - c0);
+    int32_t *Xt = X + (int64_t)(T.row0 + r0) * ldx;
+    const int32_t *B = lbb + Q.lbb_off + c0;
+    const int32_t *xa = Xt + (int64_t)arow * ldx;
+    int ra[4], rb[4];
+    auto load = [&](int k0) {
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int row = row0 + 8 * i;
-    if (row >= m_valid) continue;
-    int32_t *cr = Xt + (int64_t)row * ldx + Q.hoff + c0 + col0;
+      for (int j = 0; j < 4; j++) {
+        const int k = k0 + akq + j;
+        int v = kInf;
+        if (arow < m_valid && k < K) {
+          const int h = Q.hoff + klist[k];
+          const int e1 = in_off[h + 1];
+          for (int e = in_off[h]; e < e1; e++) {
+            const int2 a = in_arc[e];
+            v = min(v, xa[a.x] + a.y);
+          }
+        }
+        ra[j] = v;
+      }
+      const int kr = k0 + (t >> 4), col = (t & 15) * 4;
+      const int32_t *pb = B + (int64_t)(kr < K ? klist[kr] : 0) * Q.ldb + col;
+      if (kr < K && col + 4 <= n_valid) {
+        const int4 v = *reinterpret_cast<const int4 *>(pb);
+        rb[0] = v.x; rb[1] = v.y; rb[2] = v.z; rb[3] = v.w;
+      } else {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      if (col0 + c >= n_valid) continue;
-      if (acc[i][c] < cr[c]) {
-        cr[c] = acc[i][c];
-        lowest = min(lowest, acc[i][c]);
+        for (int j = 0; j < 4; j++) rb[j] = (kr < K && col + j < n_valid) ? pb[j] : kInf;
+      }
+    };
+    int acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[i][c] = kInf;
+    int32_t *As = lds;
+    int32_t *Bs = lds + 64 * MP_AS_LD;
+    const int nch = (K + MP_BK - 1) / MP_BK;
+    if (nch > 0) {
+      load(0);
+      MpSmall::store_chunk(As, Bs, ra, rb);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
+      const bool more = ch + 1 < nch;
+      if (more) load((ch + 1) * MP_BK);
+      const int krem = K - ch * MP_BK;
+      MpSmall::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
+      __syncthreads();
+      if (more) {
+        MpSmall::store_chunk(As, Bs, ra, rb);
+        __syncthreads();
       }
     }
-  }
+    int lowest = kInf;  // smallest improved value
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off));
-  if (lane == 0 && lowest < kInf) atomicMin(&flags[T.flag], lowest);
+    for (int i = 0; i < 4; i++) {
+      const int row = row0 + 8 * i;
+      if (row >= m_valid) continue;
+      int32_t *cr = Xt + (int64_t)row * ldx + Q.hoff + c0 + col0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        if (col0 + c >= n_valid) continue;
+        if (acc[i][c] < cr[c]) {
+          cr[c] = acc[i][c];
+          lowest = min(lowest, acc[i][c]);
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off));
+    if (lane == 0 && lowest < kInf) atomicMin(&tmin[T.task], lowest);
+    __syncthreads();  // klist and the staging are rewritten by the next tile
+  }
 }
 
 // Rows of the expansion: tile (64 sources, 64 columns of part q) =
@@ -310,7 +434,9 @@ namespace {
 
 void free_bidx(fap_bidx *x) {
   if (!x) return;
-  void *bufs[] = {x->parts, x->in_off, x->in_arc, x->hloc, x->lbb, x->lbv, x->colid, x->ctiles};
+  void *bufs[] = {x->parts,   x->in_off, x->in_arc, x->hloc,     x->inb_off, x->inb_pos_off,
+                  x->inb_pos, x->outb_off, x->outb,  x->lbb,      x->lbv,     x->colid,
+                  x->ctiles};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete x;
@@ -441,7 +567,7 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
     double wsum = 0;
     for (int64_t e = 0; e < g->E; e++) wsum += ed[e].y;
     const double mean_w = g->E ? wsum / (double)g->E : 1.0;
-    x->delta = (int32_t)std::min<double>(std::max(1.0, 0.5 * mean_w * std::sqrt((double)V / K)),
+    x->delta = (int32_t)std::min<double>(std::max(1.0, mean_w * std::sqrt((double)V / K)),
                                          kInf / 4);
   }
   // H order: each part's boundary vertices grouped by the neighbouring part
@@ -475,36 +601,49 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   // (q, q2) with a cut arc q -> q2, the H positions of q2 it enters
   std::vector<int32_t> in_off(x->nbp + 1, 0);
   std::vector<int2> in_arc;
-  x->nbr.assign(K, {});
-  x->bord.clear();
+  std::vector<int32_t> inb_off(K + 1, 0), inb_pos_off(1, 0), inb_pos, outb_off(K + 1, 0);
+  std::vector<int2> outb;
   {
     std::vector<std::vector<int2>> lists(x->nbp);
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> cuts(K);  // part q: {q2, position}
+    // per entered part q2: {source part q, H position of q2}
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> cuts(K);
     for (int32_t u = 0; u < V; u++)
       for (int32_t e = ro[u]; e < ro[u + 1]; e++) {
         const int32_t v = ed[e].x;
         if (gid[v] == gid[u]) continue;
         lists[hid[v]].push_back(make_int2(hid[u], ed[e].y));
-        cuts[gid[u] - 1].push_back({gid[v] - 1, hid[v] - x->h_parts[gid[v] - 1].hoff});
+        cuts[gid[v] - 1].push_back({gid[u] - 1, hid[v] - x->h_parts[gid[v] - 1].hoff});
       }
     for (int32_t h = 0; h < x->nbp; h++) {
       in_off[h + 1] = in_off[h] + (int32_t)lists[h].size();
       in_arc.insert(in_arc.end(), lists[h].begin(), lists[h].end());
     }
-    for (int32_t q = 0; q < K; q++) {
-      std::vector<std::pair<int32_t, int32_t>> &c = cuts[q];
+    std::vector<std::vector<int2>> outs(K);  // part q: {q2, slot of q in q2}
+    for (int32_t q2 = 0; q2 < K; q2++) {
+      std::vector<std::pair<int32_t, int32_t>> &c = cuts[q2];
       std::sort(c.begin(), c.end());
       c.erase(std::unique(c.begin(), c.end()), c.end());
-      for (size_t i = 0; i < c.size();) {
-        size_t j = i;
-        std::vector<int32_t> pos;
-        while (j < c.size() && c[j].first == c[i].first) pos.push_back(c[j++].second);
-        x->nbr[q].push_back(make_int2(c[i].first, (int32_t)x->bord.size()));
-        x->bord.push_back(std::move(pos));
-        i = j;
+      int32_t slot = 0;
+      for (size_t i = 0; i < c.size(); slot++) {
+        const int32_t q = c[i].first;
+        while (i < c.size() && c[i].first == q) inb_pos.push_back(c[i++].second);
+        inb_pos_off.push_back((int32_t)inb_pos.size());
+        outs[q].push_back(make_int2(q2, slot));
+      }
+      inb_off[q2 + 1] = inb_off[q2] + slot;
+      const int32_t kl = inb_pos_off.back() - inb_pos_off[inb_off[q2]];
+      if (kl > kBidxKlMax) {  // a tile's border list would not fit in LDS
+        delete x;
+        return FAP_OK;
       }
     }
+    for (int32_t q = 0; q < K; q++) {
+      outb_off[q + 1] = outb_off[q] + (int32_t)outs[q].size();
+      outb.insert(outb.end(), outs[q].begin(), outs[q].end());
+    }
   }
+  x->max_ct = 0;
+  for (int32_t q = 0; q < K; q++) x->max_ct += (x->h_parts[q].b + 63) / 64;
   std::vector<int32_t> colid(st2ed);
   std::vector<int2> ctiles;
   for (int32_t q = 0; q < K; q++) {
@@ -518,6 +657,11 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   if (rc == FAP_OK) rc = upload(in_off, &x->in_off, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(in_arc, &x->in_arc, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(hloc, &x->hloc, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(inb_off, &x->inb_off, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(inb_pos_off, &x->inb_pos_off, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(inb_pos, &x->inb_pos, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(outb_off, &x->outb_off, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(outb, &x->outb, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(colid, &x->colid, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(ctiles, &x->ctiles, &x->bytes, s);
   // (every word of both is written by the extraction below)
@@ -578,15 +722,15 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
 
 namespace {
 
-// Pinned host and device buffers of the block iterations (tiles, flags),
-// grown on demand.
-int bbf_buffers(fap_ctx *ctx, size_t bytes, char **h, char **d) {
-  if (ctx->host_bbf_bytes < bytes) {
+// Device workspace of the block iterations (`bytes`) and its pinned host
+// staging (`hbytes`, the leading part uploaded once), grown on demand.
+int bbf_buffers(fap_ctx *ctx, size_t bytes, size_t hbytes, char **h, char **d) {
+  if (ctx->host_bbf_bytes < hbytes) {
     if (ctx->host_bbf) (void)hipHostFree(ctx->host_bbf);
     ctx->host_bbf = nullptr;
     ctx->host_bbf_bytes = 0;
-    FAP_HIP(hipHostMalloc(&ctx->host_bbf, bytes, hipHostMallocDefault));
-    ctx->host_bbf_bytes = bytes;
+    FAP_HIP(hipHostMalloc(&ctx->host_bbf, hbytes, hipHostMallocDefault));
+    ctx->host_bbf_bytes = hbytes;
   }
   FAP_TRY(ensure_ws(ctx, ctx->ws_bbf, bytes));
   *h = (char *)ctx->host_bbf;
@@ -598,9 +742,10 @@ int bbf_buffers(fap_ctx *ctx, size_t bytes, char **h, char **d) {
 
 // Stage-1 rows for the boundary vertices of `count` parts (part indices
 // qidx, X rows from rowoff[p], nsrc rows in all; d_rowptr: device row
-// pointers of the output rows, V columns each, in the same order).  Runs the
-// block iterations from the host (it waits on s each iteration), then
-// enqueues the expansion on s.  Uses ctx->ws_bidx (nsrc x nbp int32).
+// pointers of the output rows, V columns each, in the same order): the
+// block iterations, then the expansion, all enqueued on s.  The host looks
+// at the done word every kBatch iterations (one wait on s each time).
+// Uses ctx->ws_bidx (nsrc x nbp int32) and ctx->ws_bbf / host_bbf.
 int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *qidx,
               const int32_t *rowoff, int32_t nsrc, int32_t *const *d_rowptr, hipStream_t s) {
   const fap_bidx *x = g->bidx;
@@ -608,24 +753,49 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
   if (nsrc <= 0) return FAP_OK;
   const int rows64 = (nsrc + 63) / 64;
   if (rows64 > 65535) return fail(FAP_E_ARG, "boundary rows: too many sources in one call");
+  const int32_t K = x->K;
+  const int64_t nblk = (int64_t)count * K;
+  if (nblk > INT32_MAX / 2) return fail(FAP_E_ARG, "boundary rows: too many parts in one call");
   const int64_t ldx = x->nbp;  // a multiple of 4
   FAP_TRY(ensure_ws(ctx, ctx->ws_bidx, (size_t)nsrc * ldx * sizeof(int32_t)));
   int32_t *X = (int32_t *)ctx->ws_bidx.ptr;
-  // blocks start as each part's own boundary distances
-  std::vector<int2> rowq(nsrc);
-  for (int32_t p = 0; p < count; p++) {
-    const int32_t q = qidx[p];
-    for (int32_t i = 0; i < x->h_parts[q].b; i++)
-      rowq[rowoff[p] + i] = make_int2(q, x->h_hpos[x->boff[q] + i]);
-  }
+  // workspace: control | block rows {first X row, rows} | source rows {part,
+  // H position} | pending | marks | tasks | task minima | tiles
+  int64_t rt_sum = 0;
+  for (int32_t p = 0; p < count; p++) rt_sum += (x->h_parts[qidx[p]].b + 63) / 64;
+  const int64_t max_tiles = std::max<int64_t>(rt_sum * x->max_ct, 1);
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t o_prow = up(sizeof(int32_t) * BC_WORDS);
+  const size_t o_rowq = o_prow + up(sizeof(int2) * count);
+  const size_t o_pend = o_rowq + up(sizeof(int2) * nsrc);
+  const size_t o_mask = o_pend + up(sizeof(int32_t) * nblk);
+  const size_t o_task = o_mask + up(sizeof(uint32_t) * nblk);
+  const size_t o_tmin = o_task + up(sizeof(int32_t) * nblk);
+  const size_t o_tile = o_tmin + up(sizeof(int32_t) * nblk);
+  const size_t total = o_tile + sizeof(BidxTile) * (size_t)max_tiles;
   char *h = nullptr, *d = nullptr;
-  FAP_TRY(bbf_buffers(ctx, std::max<size_t>(sizeof(int2) * nsrc, 1 << 20), &h, &d));
-  memcpy(h, rowq.data(), sizeof(int2) * nsrc);
-  FAP_HIP(hipMemcpyAsync(d, h, sizeof(int2) * nsrc, hipMemcpyHostToDevice, s));
+  FAP_TRY(bbf_buffers(ctx, total, o_task, &h, &d));
+  int32_t *hc = (int32_t *)h;
+  std::fill(hc, hc + BC_WORDS, 0);
+  const int32_t delta = (int32_t)std::max<int64_t>(
+      1, ctx->opt_bidx_delta > 0 ? ctx->opt_bidx_delta : x->delta);
+  hc[BC_THR] = delta;
+  int2 *hprow = (int2 *)(h + o_prow);
+  int2 *hrowq = (int2 *)(h + o_rowq);
+  int32_t *hpend = (int32_t *)(h + o_pend);
+  std::fill(hpend, hpend + nblk, kInf);
+  memset(h + o_mask, 0, sizeof(uint32_t) * nblk);
+  for (int32_t p = 0; p < count; p++) {
+    const int32_t q = qidx[p], b = x->h_parts[q].b;
+    hprow[p] = make_int2(rowoff[p], b);
+    for (int32_t i = 0; i < b; i++) hrowq[rowoff[p] + i] = make_int2(q, x->h_hpos[x->boff[q] + i]);
+    if (b > 0) hpend[(size_t)p * K + q] = 0;  // each block starts as its own part's distances
+  }
+  FAP_HIP(hipMemcpyAsync(d, h, o_task, hipMemcpyHostToDevice, s));
+  int32_t *dctrl = (int32_t *)d;
   hipLaunchKernelGGL(fapk::bidx_block_init_kernel, dim3(nsrc), dim3(256), 0, s, X, ldx, x->nbp,
-                     (const int2 *)d, x->parts, x->lbb);
+                     (const int2 *)(d + o_rowq), x->parts, x->lbb);
   FAP_CHECK_LAUNCH();
-  FAP_HIP(hipStreamSynchronize(s));  // the staging is reused below
   // Bellman-Ford over the parts, block by block: a block (p, q2) is updated
   // after a block (p, q) with a cut arc q -> q2 changed, over the border
   // positions of q2 entered from such parts q.  Changed blocks wait in a
@@ -634,94 +804,27 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
   // the current threshold, raised by delta when none does, so that most
   // blocks are propagated once their values are close to final.  Every
   // change is propagated eventually, so the result does not depend on the
-  // band; values only decrease, so the loop ends.
-  const int32_t K = x->K;
-  const int64_t delta = ctx->opt_bidx_delta > 0 ? ctx->opt_bidx_delta : x->delta;
-  std::vector<int32_t> pend((size_t)count * K, kInf);      // smallest changed value, pending blocks
-  std::vector<std::pair<int32_t, int32_t>> pending, keep, active, tasks;  // (block p, part q)
-  std::vector<std::vector<int32_t>> tbord;                 // borders of each task
-  for (int32_t p = 0; p < count; p++)
-    if (x->h_parts[qidx[p]].b > 0) {
-      pend[(size_t)p * K + qidx[p]] = 0;
-      pending.push_back({p, qidx[p]});
+  // band; values only decrease, so the iterations end.
+  const int grid = (int)std::min<int64_t>(max_tiles, 8 * std::max(ctx->num_cus, 1));
+  constexpr int kBatch = 16;
+  for (int64_t it = 0;; it += kBatch) {
+    if (it > ((int64_t)1 << 22)) return fail(FAP_E_STATE, "boundary iterations did not settle");
+    for (int i = 0; i < kBatch; i++) {
+      hipLaunchKernelGGL(fapk::bidx_schedule_kernel, dim3(1), dim3(1024), 0, s, dctrl,
+                         (int32_t *)(d + o_pend), (uint32_t *)(d + o_mask),
+                         (int32_t *)(d + o_task), (int32_t *)(d + o_tmin),
+                         (BidxTile *)(d + o_tile), (const int2 *)(d + o_prow), count, K, delta,
+                         x->parts, x->outb_off, x->outb);
+      FAP_CHECK_LAUNCH();
+      hipLaunchKernelGGL(fapk::bidx_block_kernel, dim3(grid), dim3(fapk::MP_THREADS), 0, s,
+                         (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx, x->parts,
+                         x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off, x->inb_pos,
+                         (int32_t *)(d + o_tmin));
+      FAP_CHECK_LAUNCH();
     }
-  std::vector<int32_t> tix((size_t)count * K, -1);
-  std::vector<BidxTile> tl;
-  std::vector<int32_t> kl;
-  int64_t thr = delta;
-  for (int64_t it = 0; !pending.empty(); it++) {
-    if (it > ((int64_t)1 << 24)) return fail(FAP_E_STATE, "boundary iterations did not settle");
-    int64_t lo = kInf;
-    for (const auto &pq : pending) lo = std::min<int64_t>(lo, pend[(size_t)pq.first * K + pq.second]);
-    if (lo >= thr) thr = (lo / delta + 1) * delta;
-    active.clear();
-    keep.clear();
-    for (const auto &pq : pending) {
-      int32_t &v = pend[(size_t)pq.first * K + pq.second];
-      if (v < thr) {
-        active.push_back(pq);
-        v = kInf;
-      } else {
-        keep.push_back(pq);
-      }
-    }
-    pending.swap(keep);
-    tasks.clear();
-    for (const auto &pq : active)
-      for (const int2 nb : x->nbr[pq.second]) {
-        const int32_t q2 = nb.x;
-        if (x->h_parts[q2].b == 0) continue;
-        int32_t &t = tix[(size_t)pq.first * K + q2];
-        if (t < 0) {
-          t = (int32_t)tasks.size();
-          tasks.push_back({pq.first, q2});
-          if (tbord.size() < tasks.size()) tbord.emplace_back();
-          tbord[t].clear();
-        }
-        tbord[t].push_back(nb.y);
-      }
-    for (const auto &pq : tasks) tix[(size_t)pq.first * K + pq.second] = -1;
-    if (tasks.empty()) continue;
-    tl.clear();
-    kl.clear();
-    for (int32_t j = 0; j < (int32_t)tasks.size(); j++) {
-      const int32_t p = tasks[j].first, q2 = tasks[j].second;
-      const int32_t koff = (int32_t)kl.size();
-      for (int32_t bd : tbord[j]) kl.insert(kl.end(), x->bord[bd].begin(), x->bord[bd].end());
-      if (tbord[j].size() > 1) {
-        std::sort(kl.begin() + koff, kl.end());
-        kl.erase(std::unique(kl.begin() + koff, kl.end()), kl.end());
-      }
-      const int32_t kn = (int32_t)kl.size() - koff;
-      const int32_t rows = x->h_parts[qidx[p]].b, cols = x->h_parts[q2].b;
-      for (int32_t tr = 0; tr * 64 < rows; tr++)
-        for (int32_t tc = 0; tc * 64 < cols; tc++)
-          tl.push_back(BidxTile{rowoff[p], rows, q2, tr, tc, j, koff, kn});
-    }
-    // staging: tiles | border positions | changed flags
-    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    const size_t tbytes = sizeof(BidxTile) * tl.size();
-    const size_t koffb = up(tbytes);
-    const size_t foff = koffb + up(sizeof(int32_t) * kl.size());
-    const size_t fbytes = sizeof(int32_t) * tasks.size();
-    FAP_TRY(bbf_buffers(ctx, foff + fbytes, &h, &d));
-    memcpy(h, tl.data(), tbytes);
-    memcpy(h + koffb, kl.data(), sizeof(int32_t) * kl.size());
-    std::fill((int32_t *)(h + foff), (int32_t *)(h + foff + fbytes), kInf);
-    FAP_HIP(hipMemcpyAsync(d, h, foff + fbytes, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(fapk::bidx_block_kernel, dim3((unsigned)tl.size()), dim3(fapk::MP_THREADS),
-                       0, s, (const BidxTile *)d, (const int32_t *)(d + koffb), X, ldx, x->parts,
-                       x->in_off, x->in_arc, x->lbb, (int32_t *)(d + foff));
-    FAP_CHECK_LAUNCH();
-    FAP_HIP(hipMemcpyAsync(h + foff, d + foff, fbytes, hipMemcpyDeviceToHost, s));
+    FAP_HIP(hipMemcpyAsync(h, d, sizeof(int32_t) * BC_WORDS, hipMemcpyDeviceToHost, s));
     FAP_HIP(hipStreamSynchronize(s));
-    const int32_t *fl = (const int32_t *)(h + foff);
-    for (int32_t j = 0; j < (int32_t)tasks.size(); j++) {
-      if (fl[j] >= kInf) continue;
-      int32_t &v = pend[(size_t)tasks[j].first * K + tasks[j].second];
-      if (v >= kInf) pending.push_back(tasks[j]);
-      v = std::min(v, fl[j]);
-    }
+    if (hc[BC_DONE]) break;
   }
   hipLaunchKernelGGL(fapk::bidx_expand_kernel, dim3(x->nctiles, rows64), dim3(fapk::MP_THREADS), 0,
                      s, X, ldx, nsrc, d_rowptr, x->ctiles, x->parts, x->lbv, x->colid);

@@ -89,7 +89,7 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "sssp_index"   0/1      batched part solves compute stage 1 through the graph's
  *                           boundary index (built by the first such call) [1]
  *   "bidx_delta"   >= 0     distance band of the boundary index's block iterations; any
- *                           value gives the same rows [mean arc weight x sqrt(V / K) / 2] */
+ *                           value gives the same rows [mean arc weight x sqrt(V / K)] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);
