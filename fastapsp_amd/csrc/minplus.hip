@@ -36,6 +36,19 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4
              imin(MP_BM, M - m0), imin(MP_BN, N - n0), K, accumulate != 0, flags, lds);
 }
 
+// The last rows of a product whose row count leaves at most 64 rows past
+// its last full 128-row tile: 64 x 64 tiles, one row of them, so that the
+// tail computes 64 instead of 128 rows per column panel.
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void minplus_tail64_kernel(
+    const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C, int64_t ldc,
+    int M, int N, int K, int accumulate, int flags, int tiles_n) {
+  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  const int tn = xcd_remap(blockIdx.x, tiles_n);
+  const int n0 = tn * 64;
+  MpSmall::run(A, lda, B + n0, ldb, C + n0, ldc, M, imin(64, N - n0), K, accumulate != 0, flags,
+               lds);
+}
+
 }  // namespace fapk
 
 namespace fapi {
@@ -53,14 +66,28 @@ int launch_minplus(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb,
                    int64_t ldc, int32_t M, int32_t N, int32_t K, int accumulate,
                    hipStream_t s) {
   if (M <= 0 || N <= 0) return FAP_OK;
-  const int64_t tiles_m = (M + fapk::MP_BM - 1) / fapk::MP_BM;
+  // rows past the last full 128-row tile: a 64-row tail when they fit in one
+  const int32_t rem = M % fapk::MP_BM;
+  const int32_t tail = (M > fapk::MP_BM && rem > 0 && rem <= 64) ? rem : 0;
+  const int32_t Mb = M - tail;
+  const int64_t tiles_m = (Mb + fapk::MP_BM - 1) / fapk::MP_BM;
   const int64_t tiles_n = (N + fapk::MP_BN - 1) / fapk::MP_BN;
   if (tiles_m * tiles_n > INT32_MAX) return fail(FAP_E_ARG, "min-plus: too many tiles");
   const int flags = vec_flags(A, lda, B, ldb, C, ldc);
   hipLaunchKernelGGL(fapk::minplus_gemm_kernel, dim3((unsigned)(tiles_m * tiles_n)),
-                     dim3(fapk::MP_THREADS), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N,
+                     dim3(fapk::MP_THREADS), 0, s, A, lda, B, ldb, C, ldc, (int)Mb, (int)N,
                      (int)K, accumulate, flags, (int)tiles_m, (int)tiles_n);
   FAP_CHECK_LAUNCH();
+  if (tail > 0) {
+    const int64_t tiles_n64 = (N + 63) / 64;
+    const int32_t *At = A + (int64_t)Mb * lda;
+    int32_t *Ct = C + (int64_t)Mb * ldc;
+    hipLaunchKernelGGL(fapk::minplus_tail64_kernel, dim3((unsigned)tiles_n64),
+                       dim3(fapk::MP_THREADS), 0, s, At, lda, B, ldb, Ct, ldc, (int)tail, (int)N,
+                       (int)K, accumulate, vec_flags(At, lda, B, ldb, Ct, ldc),
+                       (int)tiles_n64);
+    FAP_CHECK_LAUNCH();
+  }
   return FAP_OK;
 }
 
