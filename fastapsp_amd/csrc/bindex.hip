@@ -73,8 +73,9 @@ struct BidxPart {
   int32_t ldv;      // n rounded up to a multiple of 4
 };
 
-// One 64 x 64 tile of a block update: rows tr*64.. of the block whose first
-// row of X is row0 (rows rows), columns tc*64.. of part q's boundary; the
+// One tile (64 rows x 64 or 128 columns) of a block update: rows tr*64.. of
+// the block whose first row of X is row0 (rows rows), columns tc*64.. or
+// tc*128.. of part q's boundary; the
 // product runs over the border positions of q entered from the parts
 // whose bits are set in `slots` (bit s: q's s-th in-neighbour part; bit 31
 // stands for every in-neighbour from the 31st on).  `task` indexes the
@@ -101,7 +102,6 @@ struct fap_bidx {
   std::vector<BidxPart> h_parts;          // host copy, index = part id - 1
   std::vector<int32_t> h_hpos;            // H position of each boundary vertex (local index order)
   std::vector<int32_t> boff;              // K + 1: each part's first entry in h_hpos
-  int32_t max_ct = 0;                     // sum over parts of their 64-column tiles
   BidxPart *parts = nullptr;       // device, K
   int32_t *in_off = nullptr;       // nbp + 1: cut arcs into each H id
   int2 *in_arc = nullptr;          // {source H id, weight}
@@ -147,13 +147,13 @@ __global__ void bidx_block_init_kernel(int32_t *__restrict__ X, int64_t ldx, int
 // smallest pending value when none lies below it, takes every pending
 // block below the threshold (clearing it) and marks, for each part q2 a
 // cut arc from its part q enters, the slot of q among q2's in-neighbours in
-// mask[p * K + q2]; then lists the marked blocks as tasks and their 64 x 64
-// tiles.  With nothing pending it sets the done word and lists nothing, so
+// mask[p * K + q2]; then lists the marked blocks as tasks and their
+// 64 x tcol tiles.  With nothing pending it sets the done word and lists nothing, so
 // launches enqueued after convergence are no-ops.
 __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
     int32_t *ctrl, int32_t *pend, uint32_t *mask, int32_t *tasks, int32_t *tmin,
     BidxTile *tiles, const int2 *__restrict__ prow, int32_t count, int32_t K, int32_t delta,
-    const BidxPart *__restrict__ parts, const int32_t *__restrict__ outb_off,
+    int32_t tcol, const BidxPart *__restrict__ parts, const int32_t *__restrict__ outb_off,
     const int2 *__restrict__ outb) {
   __shared__ int s_red[16];
   __shared__ int s_thr, s_nt, s_ntl;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
     tasks[t] = i;
     tmin[t] = kInf;
     const int2 pr = prow[p];
-    const int rt = (pr.y + 63) >> 6, ct = (parts[q2].b + 63) >> 6;
+    const int rt = (pr.y + 63) >> 6, ct = (parts[q2].b + tcol - 1) / tcol;
     const int base = atomicAdd(&s_ntl, rt * ct);
     for (int tr = 0; tr < rt; tr++)
       for (int tc = 0; tc < ct; tc++)
@@ -237,20 +237,23 @@ __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
 // values read by the gather may be lowered concurrently by other tiles;
 // either value is the length of a walk, and a lowered one is pending again,
 // so the iteration only converges on exact distances.
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void bidx_block_kernel(
+template <int TM, int TN>
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN == 4 ? 5 : 4))) void bidx_block_kernel(
     const int32_t *__restrict__ ctrl, const BidxTile *__restrict__ tiles, int32_t *X, int64_t ldx,
     const BidxPart *__restrict__ parts, const int32_t *__restrict__ in_off,
     const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb,
     const int32_t *__restrict__ inb_off, const int32_t *__restrict__ inb_pos_off,
     const int32_t *__restrict__ inb_pos, int32_t *tmin) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  using Tile = MpTile<TM, TN>;
+  constexpr int TR = Tile::BM, TC = Tile::BN;  // rows, columns of a tile
+  __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
   __shared__ int32_t klist[kBidxKlMax];
   const int ntiles = ctrl[BC_NTILES];
   const int t = threadIdx.x;
-  const int arow = t >> 2, akq = (t & 3) * 4;  // A staging: row, first k (MpSmall's layout)
+  const int arow = t / Tile::A_TPR, akq = (t % Tile::A_TPR) * TM;  // A staging (the tile's layout)
   const int wave = t >> 6, lane = t & 63;
-  const int row0 = (wave >> 1) * 32 + (lane >> 3);
-  const int col0 = (wave & 1) * 32 + (lane & 7) * 4;
+  const int row0 = (wave >> 1) * (8 * TM) + (lane >> 3);  // the tile's output layout
+  const int col0 = (wave & 1) * (8 * TN) + (lane & 7) * 4;
   for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
     const BidxTile T = tiles[ti];
     const BidxPart Q = parts[T.q];
@@ -267,15 +270,15 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
     }
     const int K = kn;
     __syncthreads();
-    const int r0 = T.tr * 64, c0 = T.tc * 64;
-    const int m_valid = min(64, T.rows - r0), n_valid = min(64, Q.b - c0);
+    const int r0 = T.tr * TR, c0 = T.tc * TC;
+    const int m_valid = min(TR, T.rows - r0), n_valid = min(TC, Q.b - c0);
     int32_t *Xt = X + (int64_t)(T.row0 + r0) * ldx;
     const int32_t *B = lbb + Q.lbb_off + c0;
     const int32_t *xa = Xt + (int64_t)arow * ldx;
-    int ra[4], rb[4];
+    int ra[TM], rb[TN];
     auto load = [&](int k0) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < TM; j++) {
         const int k = k0 + akq + j;
         int v = kInf;
         if (arow < m_valid && k < K) {
@@ -288,54 +291,60 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
         }
         ra[j] = v;
       }
-      const int kr = k0 + (t >> 4), col = (t & 15) * 4;
+      const int kr = k0 + (t >> 4), col = (t & 15) * TN;
       const int32_t *pb = B + (int64_t)(kr < K ? klist[kr] : 0) * Q.ldb + col;
-      if (kr < K && col + 4 <= n_valid) {
-        const int4 v = *reinterpret_cast<const int4 *>(pb);
-        rb[0] = v.x; rb[1] = v.y; rb[2] = v.z; rb[3] = v.w;
+      if (kr < K && col + TN <= n_valid) {
+#pragma unroll
+        for (int q = 0; q < TN; q += 4) {
+          const int4 v = *reinterpret_cast<const int4 *>(pb + q);
+          rb[q] = v.x; rb[q + 1] = v.y; rb[q + 2] = v.z; rb[q + 3] = v.w;
+        }
       } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) rb[j] = (kr < K && col + j < n_valid) ? pb[j] : kInf;
+        for (int j = 0; j < TN; j++) rb[j] = (kr < K && col + j < n_valid) ? pb[j] : kInf;
       }
     };
-    int acc[4][4];
+    int acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) acc[i][c] = kInf;
+      for (int c = 0; c < TN; c++) acc[i][c] = kInf;
     int32_t *As = lds;
-    int32_t *Bs = lds + 64 * MP_AS_LD;
+    int32_t *Bs = lds + TR * MP_AS_LD;
     const int nch = (K + MP_BK - 1) / MP_BK;
     if (nch > 0) {
       load(0);
-      MpSmall::store_chunk(As, Bs, ra, rb);
+      Tile::store_chunk(As, Bs, ra, rb);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
       const bool more = ch + 1 < nch;
       if (more) load((ch + 1) * MP_BK);
       const int krem = K - ch * MP_BK;
-      MpSmall::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
+      Tile::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
       __syncthreads();
       if (more) {
-        MpSmall::store_chunk(As, Bs, ra, rb);
+        Tile::store_chunk(As, Bs, ra, rb);
         __syncthreads();
       }
     }
     int lowest = kInf;  // smallest improved value
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < TM; i++) {
       const int row = row0 + 8 * i;
       if (row >= m_valid) continue;
-      int32_t *cr = Xt + (int64_t)row * ldx + Q.hoff + c0 + col0;
+      int32_t *cr = Xt + (int64_t)row * ldx + Q.hoff + c0;
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        if (col0 + c >= n_valid) continue;
-        if (acc[i][c] < cr[c]) {
-          cr[c] = acc[i][c];
-          lowest = min(lowest, acc[i][c]);
+      for (int h = 0; h < TN / 4; h++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int col = col0 + 32 * h + c;
+          if (col >= n_valid) continue;
+          if (acc[i][4 * h + c] < cr[col]) {
+            cr[col] = acc[i][4 * h + c];
+            lowest = min(lowest, acc[i][4 * h + c]);
+          }
         }
-      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off));
@@ -642,8 +651,6 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
       outb.insert(outb.end(), outs[q].begin(), outs[q].end());
     }
   }
-  x->max_ct = 0;
-  for (int32_t q = 0; q < K; q++) x->max_ct += (x->h_parts[q].b + 63) / 64;
   std::vector<int32_t> colid(st2ed);
   std::vector<int2> ctiles;
   for (int32_t q = 0; q < K; q++) {
@@ -761,9 +768,12 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
   int32_t *X = (int32_t *)ctx->ws_bidx.ptr;
   // workspace: control | block rows {first X row, rows} | source rows {part,
   // H position} | pending | marks | tasks | task minima | tiles
-  int64_t rt_sum = 0;
+  // block-update tile: 64 rows x tcol columns (tcol = 64 or 128)
+  const int tcol = ctx->opt_bidx_tile == 64 ? 64 : 128;
+  int64_t rt_sum = 0, ct_sum = 0;
   for (int32_t p = 0; p < count; p++) rt_sum += (x->h_parts[qidx[p]].b + 63) / 64;
-  const int64_t max_tiles = std::max<int64_t>(rt_sum * x->max_ct, 1);
+  for (int32_t q = 0; q < K; q++) ct_sum += (x->h_parts[q].b + tcol - 1) / tcol;
+  const int64_t max_tiles = std::max<int64_t>(rt_sum * ct_sum, 1);
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t o_prow = up(sizeof(int32_t) * BC_WORDS);
   const size_t o_rowq = o_prow + up(sizeof(int2) * count);
@@ -813,13 +823,19 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
       hipLaunchKernelGGL(fapk::bidx_schedule_kernel, dim3(1), dim3(1024), 0, s, dctrl,
                          (int32_t *)(d + o_pend), (uint32_t *)(d + o_mask),
                          (int32_t *)(d + o_task), (int32_t *)(d + o_tmin),
-                         (BidxTile *)(d + o_tile), (const int2 *)(d + o_prow), count, K, delta,
+                         (BidxTile *)(d + o_tile), (const int2 *)(d + o_prow), count, K, delta, tcol,
                          x->parts, x->outb_off, x->outb);
       FAP_CHECK_LAUNCH();
-      hipLaunchKernelGGL(fapk::bidx_block_kernel, dim3(grid), dim3(fapk::MP_THREADS), 0, s,
-                         (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx, x->parts,
-                         x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off, x->inb_pos,
-                         (int32_t *)(d + o_tmin));
+      if (tcol == 64)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 4>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
+                           (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
+                           x->parts, x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
+                           x->inb_pos, (int32_t *)(d + o_tmin));
+      else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 8>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
+                           (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
+                           x->parts, x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
+                           x->inb_pos, (int32_t *)(d + o_tmin));
       FAP_CHECK_LAUNCH();
     }
     FAP_HIP(hipMemcpyAsync(h, d, sizeof(int32_t) * BC_WORDS, hipMemcpyDeviceToHost, s));

@@ -108,6 +108,7 @@ struct fap_ctx {
   int opt_sssp_wgs = 0;          // SSSP workgroups per CU: 1..8; 0 = 1024 threads per CU
   int opt_sssp_arcs = 2;         // arcs per lane per SSSP relaxation step: 1, 2, 4
   int opt_sssp_index = 1;        // batched part solves: stage 1 through the boundary index (1) or SSSP (0)
+  int opt_bidx_tile = 0;         // boundary iterations' tile columns: 64 or 128; 0 = 128
   int opt_bidx_delta = 0;        // distance band of the boundary iterations; 0 = mean arc weight x sqrt(V / K)
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse

@@ -256,6 +256,10 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "sssp_index") {
     if (value < 0 || value > 1) return fail(FAP_E_ARG, "sssp_index must be 0 or 1");
     ctx->opt_sssp_index = (int)value;
+  } else if (k == "bidx_tile") {
+    if (value != 0 && value != 64 && value != 128)
+      return fail(FAP_E_ARG, "bidx_tile must be 0 (auto), 64 or 128");
+    ctx->opt_bidx_tile = (int)value;
   } else if (k == "bidx_delta") {
     if (value < 0 || value > FAP_INF / 4) return fail(FAP_E_ARG, "bidx_delta out of range");
     ctx->opt_bidx_delta = (int)value;
@@ -283,6 +287,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_arcs") *value = ctx->opt_sssp_arcs;
   else if (k == "sssp_index") *value = ctx->opt_sssp_index;
   else if (k == "bidx_delta") *value = ctx->opt_bidx_delta;
+  else if (k == "bidx_tile") *value = ctx->opt_bidx_tile;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

@@ -284,6 +284,7 @@ int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, con
   fap_ctx *peer = ctx->peer;
   peer->opt_sssp_index = ctx->opt_sssp_index;
   peer->opt_bidx_delta = ctx->opt_bidx_delta;
+  peer->opt_bidx_tile = ctx->opt_bidx_tile;
   peer->opt_fw_lookahead = ctx->opt_fw_lookahead;
   peer->opt_fw_tile = ctx->opt_fw_tile;
   peer->opt_sssp_threads = ctx->opt_sssp_threads;

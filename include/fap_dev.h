@@ -88,6 +88,7 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64]
  *   "sssp_index"   0/1      batched part solves compute stage 1 through the graph's
  *                           boundary index (built by the first such call) [1]
+ *   "bidx_tile"    0/64/128 columns of the boundary index's 64-row block-update tiles [128]
  *   "bidx_delta"   >= 0     distance band of the boundary index's block iterations; any
  *                           value gives the same rows [mean arc weight x sqrt(V / K)] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);

@@ -467,21 +467,28 @@ def test_boundary_index_rows_equal_sssp_rows(gpu_ctx, n, K, wl, wh):
     assert nb == int(pc["bdy_num"][1:].sum()) and (nbytes > 0) == (K > 1)
     ids = list(range(1, K + 1))
     got = {}
-    for flag in (1, 0):
-        gpu_ctx.set_option("sssp_index", flag)
+    # stage 1 through the index with 64x128 and 64x64 block tiles and a
+    # narrow band, and through SSSP
+    for cfg in ((1, 0, 0), (1, 64, 0), (1, 128, 50), (0, 0, 0)):
+        gpu_ctx.set_option("sssp_index", cfg[0])
+        gpu_ctx.set_option("bidx_tile", cfg[1])
+        gpu_ctx.set_option("bidx_delta", cfg[2])
         try:
             outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), n)) for p in ids]
             dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in ids],
                                     [int(pc["ver_num"][p]) for p in ids],
                                     [int(pc["bdy_num"][p]) for p in ids], [o.ptr for o in outs], n)
-            got[flag] = [o.get() for o in outs]
+            got[cfg] = [o.get() for o in outs]
             for o in outs:
                 o.free()
         finally:
             gpu_ctx.set_option("sssp_index", 1)
+            gpu_ctx.set_option("bidx_tile", 0)
+            gpu_ctx.set_option("bidx_delta", 0)
     for i, p in enumerate(ids):
-        np.testing.assert_array_equal(got[1][i], got[0][i])
-        np.testing.assert_array_equal(got[1][i], O.run_part(p, ro, col, wt, gid, pc))
+        want = O.run_part(p, ro, col, wt, gid, pc)
+        for cfg, rows in got.items():
+            np.testing.assert_array_equal(rows[i], want, err_msg=f"part {p}, {cfg}")
     dg.close()
 
 
