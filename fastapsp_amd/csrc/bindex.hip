@@ -93,11 +93,14 @@ enum { BC_THR = 0, BC_NTASKS = 1, BC_NTILES = 2, BC_DONE = 3, BC_WORDS = 16 };
 // in-border lengths); a graph with a longer one gets no index.
 constexpr int kBidxKlMax = 2048;
 
+// Expansion tiles: 16 kExpandTM sources x kExpandCols columns of a part.
+constexpr int kExpandTM = 8, kExpandTN = 8, kExpandCols = 16 * kExpandTN;
+
 struct fap_bidx {
   int32_t K = 0;
   int32_t nbp = 0;         // H ids, padding included
   int32_t nb = 0;          // boundary vertices
-  int32_t nctiles = 0;     // expansion column tiles (64 columns of one part)
+  int32_t nctiles = 0;     // expansion column tiles (kExpandCols columns of one part)
   int32_t delta = 1;       // distance band of the block iterations
   std::vector<BidxPart> h_parts;          // host copy, index = part id - 1
   std::vector<int32_t> h_hpos;            // H position of each boundary vertex (local index order)
@@ -353,64 +356,73 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
   }
 }
 
-// Rows of the expansion: tile (64 sources, 64 columns of part q) =
+// Rows of the expansion: tile (16 TM sources, 16 TN columns of part q) =
 // min over c < b_q of X[s][hoff_q + c] + LBV_q[c][j], written to
 // rowptr[s][colid[start_q + j]].  Out-of-range sources, columns and c read
 // INF; a part without boundary vertices gives INF (it is cut off).
-__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void bidx_expand_kernel(
+template <int TM, int TN>
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN == 4 ? 8 : 4))) void bidx_expand_kernel(
     const int32_t *__restrict__ X, int64_t ldx, int32_t nsrc, int32_t *const *__restrict__ rowptr,
     const int2 *__restrict__ ctiles, const BidxPart *__restrict__ parts,
     const int32_t *__restrict__ lbv, const int32_t *__restrict__ colid) {
-  __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
+  using Tile = MpTile<TM, TN>;
+  constexpr int TR = Tile::BM, TC = Tile::BN;
+  __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
   const int2 ct = ctiles[blockIdx.x];
   const BidxPart P = parts[ct.x];
   const int j0 = ct.y;
-  const int r0 = blockIdx.y * 64;
-  const int m_valid = min(64, nsrc - r0), n_valid = min(64, P.n - j0), K = P.b;
+  const int r0 = blockIdx.y * TR;
+  const int m_valid = min(TR, nsrc - r0), n_valid = min(TC, P.n - j0), K = P.b;
   const int32_t *A = X + (int64_t)r0 * ldx + P.hoff;
   const int32_t *B = lbv + P.lbv_off + j0;
   const int flags = MP_VEC_A | MP_VEC_B;  // hoff, ldx, lbv_off, ldv, j0 are multiples of 4
-  int acc[4][4];
+  int acc[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc[i][c] = kInf;
-  int ra[4], rb[4];
+    for (int c = 0; c < TN; c++) acc[i][c] = kInf;
+  int ra[TM], rb[TN];
   const int nch = (K + MP_BK - 1) / MP_BK;
   int32_t *As = lds;
-  int32_t *Bs = lds + 64 * MP_AS_LD;
+  int32_t *Bs = lds + TR * MP_AS_LD;
   if (nch > 0) {
-    MpSmall::load_chunk(A, ldx, B, P.ldv, m_valid, n_valid, K, 0, flags, ra, rb);
-    MpSmall::store_chunk(As, Bs, ra, rb);
+    Tile::load_chunk(A, ldx, B, P.ldv, m_valid, n_valid, K, 0, flags, ra, rb);
+    Tile::store_chunk(As, Bs, ra, rb);
   }
   __syncthreads();
   for (int ch = 0; ch < nch; ch++) {
     const bool more = ch + 1 < nch;
-    if (more) MpSmall::load_chunk(A, ldx, B, P.ldv, m_valid, n_valid, K, (ch + 1) * MP_BK, flags,
-                                  ra, rb);
+    if (more) Tile::load_chunk(A, ldx, B, P.ldv, m_valid, n_valid, K, (ch + 1) * MP_BK, flags,
+                               ra, rb);
     const int krem = K - ch * MP_BK;
-    MpSmall::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
+    Tile::compute(As, Bs, krem >= MP_BK ? MP_BK : (krem + 1) & ~1, acc);
     __syncthreads();
     if (more) {
-      MpSmall::store_chunk(As, Bs, ra, rb);
+      Tile::store_chunk(As, Bs, ra, rb);
       __syncthreads();
     }
   }
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const int row0 = (wave >> 1) * 32 + (lane >> 3);
-  const int col0 = (wave & 1) * 32 + (lane & 7) * 4;
+  const int row0 = (wave >> 1) * (8 * TM) + (lane >> 3);  // the tile's output layout
+  const int col0 = (wave & 1) * (8 * TN) + (lane & 7) * 4;
   const int32_t *ids = colid + P.start + j0;
-  int cid[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) cid[c] = col0 + c < n_valid ? ids[col0 + c] : 0;
+  for (int h = 0; h < TN / 4; h++) {
+    int cid[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int row = row0 + 8 * i;
-    if (row >= m_valid) continue;
-    int32_t *out = rowptr[r0 + row];
+    for (int c = 0; c < 4; c++) {
+      const int col = col0 + 32 * h + c;
+      cid[c] = col < n_valid ? ids[col] : -1;
+    }
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      if (col0 + c < n_valid) out[cid[c]] = acc[i][c];
+    for (int i = 0; i < TM; i++) {
+      const int row = row0 + 8 * i;
+      if (row >= m_valid) continue;
+      int32_t *out = rowptr[r0 + row];
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (cid[c] >= 0) out[cid[c]] = acc[i][4 * h + c];
+    }
   }
 }
 
@@ -656,7 +668,7 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   for (int32_t q = 0; q < K; q++) {
     const BidxPart &P = x->h_parts[q];
     std::sort(colid.begin() + P.start, colid.begin() + P.start + P.n);
-    for (int32_t j = 0; j < P.n; j += 64) ctiles.push_back(make_int2(q, j));
+    for (int32_t j = 0; j < P.n; j += kExpandCols) ctiles.push_back(make_int2(q, j));
   }
   x->nctiles = (int32_t)ctiles.size();
   int rc = FAP_OK;
@@ -842,8 +854,10 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
     FAP_HIP(hipStreamSynchronize(s));
     if (hc[BC_DONE]) break;
   }
-  hipLaunchKernelGGL(fapk::bidx_expand_kernel, dim3(x->nctiles, rows64), dim3(fapk::MP_THREADS), 0,
-                     s, X, ldx, nsrc, d_rowptr, x->ctiles, x->parts, x->lbv, x->colid);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_expand_kernel<kExpandTM, kExpandTN>),
+                     dim3(x->nctiles, (nsrc + 16 * kExpandTM - 1) / (16 * kExpandTM)),
+                     dim3(fapk::MP_THREADS), 0, s, X, ldx, nsrc, d_rowptr, x->ctiles, x->parts,
+                     x->lbv, x->colid);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }
