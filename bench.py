@@ -445,8 +445,10 @@ def grid_section(args, grid, modes=("plain", "pipelined")):
     """c3 on the grid: (1) one SSSP launch over the boundary vertices of the
     first --group parts (edges/s, algorithmic bytes 4(V+1) + 8E + 4 S V per
     launch, their rate as a fraction of HBM peak); (2) all parts solved
-    plain (fap_subgraph_solve part after part), batched (groups of --group
-    parts, fap_subgraph_solve_batch) and/or pipelined (groups alternating
+    plain (fap_subgraph_solve part after part: SSSP stage 1, the reference's
+    pipeline), single (one fap_subgraph_solve_batch per part, stage 1 through
+    the boundary index -- fapGraph::solveSubGraph's default), batched (groups
+    of --group parts, fap_subgraph_solve_batch) and/or pipelined (groups alternating
     over two streams, fap_subgraph_solve_groups -- fapGraph::solve's
     default), in calls of --call-parts parts whose ~1.1 GiB row blocks stay
     allocated; row blocks are checksummed on the device after each call
@@ -496,6 +498,10 @@ def grid_section(args, grid, modes=("plain", "pipelined")):
             if mode == "plain":
                 for i, p in enumerate(call):
                     dg.subgraph_solve(int(vo[p]), int(vn[p]), int(bn[p]), blks[i].ptr, V)
+            elif mode == "single":
+                for i, p in enumerate(call):
+                    dg.subgraph_solve_batch([int(vo[p])], [int(vn[p])], [int(bn[p])],
+                                            [blks[i].ptr], V)
             elif mode == "batched":
                 for j in range(0, len(call), G):
                     st, ns, bs, _ = args_of(call[j:j + G])
@@ -541,7 +547,7 @@ def grid_section(args, grid, modes=("plain", "pipelined")):
 
 def bench_grid_batch(args):
     """c3 alone: plain vs batched vs pipelined per-part solve on the grid."""
-    out = grid_section(args, grid_setup(args), modes=("plain", "batched", "pipelined"))
+    out = grid_section(args, grid_setup(args), modes=("plain", "single", "batched", "pipelined"))
     res = {"metric": "min-plus GOp/s", "value": out["pipelined_GOp_s"], "unit": "GOp/s",
            "n_gpus": 1, "steps": args.grid_steps, "warmup": 1,
            "ms_per_step": out["pipelined_ms"], "higher_is_better": True, "scaling": "weak",

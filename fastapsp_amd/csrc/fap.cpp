@@ -424,7 +424,7 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool is_path_needed_) {
   }
   d.block_id = 0;
   d.block_path_valid = false;
-  if (run(id, d.d_block) != 0) return -1;
+  if ((batched_ ? run_batch({id}, {d.d_block}) : run(id, d.d_block)) != 0) return -1;
   if (fap_ctx_sync(d.ctx) != FAP_OK) return fail(FAP_E_HIP, fap_last_error());
   if (is_path_needed) {
     if (d.grow(d.d_block_path, d.block_path_bytes, bytes) != FAP_OK)

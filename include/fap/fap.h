@@ -137,9 +137,12 @@ class fapGraph {
                     int *adj_size, int *row_offset, int *col_val, float *weight, int *graph_id);
 
   // ---- additions ----
-  // Batched (default) or plain part loop for solve() and solveSubGraphs():
-  // batched runs one SSSP launch for all boundary vertices, one batched
-  // closure for all part matrices; plain runs the parts one after another.
+  // Batched (default) or plain part solves for solve(), solveSubGraphs() and
+  // solveSubGraph(): batched takes the boundary rows of every part of a
+  // group from the graph's boundary index (built by the first such call)
+  // and closes the group's part matrices in one batched closure; plain runs
+  // each part as the reference does (SSSP from its boundary vertices, FW,
+  // min-plus), one after another.  Both give the same distances.
   void setBatched(bool batched) { batched_ = batched; }
   bool isBatched() const { return batched_; }
   // Several parts at once (1-based ids); each block stays on the device and

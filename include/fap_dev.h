@@ -90,7 +90,8 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *                           boundary index (built by the first such call) [1]
  *   "bidx_tile"    0/64/128 columns of the boundary index's 64-row block-update tiles [128]
  *   "bidx_delta"   >= 0     distance band of the boundary index's block iterations; any
- *                           value gives the same rows [mean arc weight x sqrt(V / K)] */
+ *                           value gives the same rows [mean arc weight x sqrt(V / K),
+ *                           times 32 / parts for calls of fewer than 32 parts] */
 int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value);
 int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value);
 int fap_ctx_sync(fap_ctx *ctx);
