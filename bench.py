@@ -258,7 +258,7 @@ def bench_dense(args):
         log(f"[bench] serve_c5 in {time.time() - t0:.1f} s: {res['serve_c5']}")
         rc |= 0 if res["serve_c5"]["identical"] else 1
         t0 = time.time()
-        res["grid_c3"] = grid_section(args, grid)
+        res["grid_c3"] = grid_section(args, grid, modes=("plain", "single", "pipelined"))
         log(f"[bench] grid_c3 in {time.time() - t0:.1f} s: {res['grid_c3']}")
         rc |= 0 if res["grid_c3"]["identical"] else 1
     else:
