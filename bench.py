@@ -444,7 +444,8 @@ def grid_setup(args):
 def grid_section(args, grid, modes=("plain", "pipelined")):
     """c3 on the grid: (1) one SSSP launch over the boundary vertices of the
     first --group parts (edges/s, algorithmic bytes 4(V+1) + 8E + 4 S V per
-    launch, their rate as a fraction of HBM peak); (2) all parts solved
+    launch, their rate as a fraction of HBM peak), and the same rows through
+    the boundary index (fap_subgraph_boundary_rows); (2) all parts solved
     plain (fap_subgraph_solve part after part: SSSP stage 1, the reference's
     pipeline), single (one fap_subgraph_solve_batch per part, stage 1 through
     the boundary index -- fapGraph::solveSubGraph's default), batched (groups
@@ -479,6 +480,18 @@ def grid_section(args, grid, modes=("plain", "pipelined")):
             "algorithmic_GB_per_s": round(sssp_bytes / (sssp_ms * 1e-3) / 1e9, 2),
             "frac_of_hbm_peak": round(sssp_bytes / (sssp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "tuned": tuned}
+    # the same sources' rows through the boundary index (stage 1 of the
+    # batched forms), into the first parts' row blocks
+    grp = parts[:G]
+    rbs = [ctx.alloc((int(bn[p]), V)) for p in grp]
+    bargs = ([int(vo[p]) for p in grp], [int(vn[p]) for p in grp], [int(bn[p]) for p in grp],
+             [b.ptr for b in rbs])
+    dg.boundary_rows(*bargs, V)          # builds the index (timed below without it)
+    idx_ms = _time_ms(lib, ctx, lambda: dg.boundary_rows(*bargs, V), reps=3)
+    for b in rbs:
+        b.free()
+    sssp["index_ms_per_call"] = round(idx_ms, 3)
+    sssp["index_over_sssp"] = round(sssp_ms / idx_ms, 2)
     log(f"[bench] c3 SSSP: {sssp}")
     P = max(G, min(args.call_parts, len(parts)))
     calls = [parts[i:i + P] for i in range(0, len(parts), P)]

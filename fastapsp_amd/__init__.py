@@ -149,6 +149,8 @@ _SIGS = {
     "fap_graph_build_index": (_int, [_vp, _vp, C.POINTER(_i32)]),
     "fap_graph_index_info": (_int, [_vp, C.POINTER(_i32), C.POINTER(_i64)]),
     "fap_subgraph_solve_groups": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, _vp, _i64, _i32]),
+    "fap_subgraph_boundary_rows": (_int, [_vp, _vp, _i32, _pi32, _pi32, _pi32, C.POINTER(_vp),
+                                          _i64, _vp]),
     "fap_subgraph_pred": (_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp]),
     "fapg_check_result": (_int, [_vp, _i32, C.POINTER(_int)]),
     # serving (include/fap_serve.h)
@@ -542,6 +544,17 @@ class DeviceGraph:
         check(load().fap_subgraph_solve_batch(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
                                               _p32(B), R, ld_rb, stream),
               "fap_subgraph_solve_batch")
+
+    def boundary_rows(self, starts, n_subs, n_bdys, rowblocks, ld_rb, stream=None):
+        """Stage 1 alone: rows 0..b-1 of each part's row block."""
+        k = len(starts)
+        S = np.ascontiguousarray(starts, dtype=np.int32)
+        N = np.ascontiguousarray(n_subs, dtype=np.int32)
+        B = np.ascontiguousarray(n_bdys, dtype=np.int32)
+        R = (_vp * k)(*rowblocks)
+        check(load().fap_subgraph_boundary_rows(self.ctx.handle, self.handle, k, _p32(S), _p32(N),
+                                                _p32(B), R, ld_rb, stream),
+              "fap_subgraph_boundary_rows")
 
     def subgraph_solve_groups(self, starts, n_subs, n_bdys, rowblocks, ld_rb, group_parts):
         k = len(starts)

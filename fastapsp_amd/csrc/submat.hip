@@ -264,6 +264,51 @@ int run_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
   return FAP_OK;
 }
 
+// Stage 1 alone for `count` parts: rows 0..b-1 of each part's row block
+// (distances from its boundary vertices to every vertex), through the
+// boundary index when it holds every part, else one SSSP launch over all
+// sources.  Synchronous.
+int run_boundary_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *starts,
+                      const int32_t *n_subs, const int32_t *n_bdys, int32_t *const *rowblocks,
+                      int64_t ld_rb, hipStream_t s) {
+  FAP_TRY(prepare_index(ctx, g));
+  int64_t nsrc = 0;
+  for (int32_t p = 0; p < count; p++) nsrc += n_bdys[p];
+  if (nsrc == 0) return FAP_OK;
+  if (nsrc > INT32_MAX) return fail(FAP_E_ARG, "boundary rows: too many sources");
+  std::vector<int32_t> qidx(count, -1), rowoff(count, 0);
+  bool use_index = ctx->opt_sssp_index && g->bidx && (nsrc + 63) / 64 <= 65535;
+  for (int32_t p = 0, r = 0; p < count && use_index; r += n_bdys[p], p++) {
+    qidx[p] = bidx_part(g, starts[p], n_subs[p], n_bdys[p]);
+    rowoff[p] = r;
+    use_index = qidx[p] >= 0;
+  }
+  const size_t o_rp = (4 * (size_t)nsrc + 255) & ~(size_t)255;
+  const size_t total = o_rp + sizeof(int32_t *) * (size_t)nsrc;
+  FAP_TRY(ensure_ws(ctx, ctx->ws_srcs, total));
+  void *hs = nullptr;
+  FAP_TRY(acquire_host_stage(ctx, total, &hs));
+  char *h = (char *)hs;
+  int32_t *h_src = (int32_t *)h;
+  int32_t **h_rp = (int32_t **)(h + o_rp);
+  for (int32_t p = 0, k = 0; p < count; p++)
+    for (int32_t i = 0; i < n_bdys[p]; i++, k++) {
+      h_src[k] = g->h_st2ed[starts[p] + i];
+      h_rp[k] = rowblocks[p] + (int64_t)i * ld_rb;
+    }
+  char *d = (char *)ctx->ws_srcs.ptr;
+  FAP_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
+  FAP_TRY(release_host_stage(ctx, s));
+  if (use_index)
+    FAP_TRY(bidx_rows(ctx, g, count, qidx.data(), rowoff.data(), (int32_t)nsrc,
+                      (int32_t *const *)(d + o_rp), s));
+  else
+    FAP_TRY(run_sssp_rows(ctx, g, (const int32_t *)d, (int32_t)nsrc, nullptr, 0,
+                          (int32_t *const *)(d + o_rp), s));
+  FAP_HIP(hipStreamSynchronize(s));
+  return FAP_OK;
+}
+
 // Groups of `group_parts` parts, alternating between ctx's stream and its
 // peer context's: one group's SSSP (HBM-bound) runs beside the other's
 // closure and products (VALU-bound).  Each group's SSSP waits for the
@@ -332,6 +377,24 @@ int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
   }
   return run_subgraph_solve_batch(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb,
                                   pick_stream(ctx, stream));
+}
+
+int fap_subgraph_boundary_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                               const int32_t *starts, const int32_t *n_subs,
+                               const int32_t *n_bdys, int32_t *const *rowblocks, int64_t ld_rb,
+                               void *stream) {
+  if (!ctx || !g) return fail(FAP_E_ARG, "null ctx/graph");
+  if (count < 0 || (count > 0 && (!starts || !n_subs || !n_bdys || !rowblocks)))
+    return fail(FAP_E_ARG, "boundary rows: bad arguments");
+  if (ld_rb < g->V) return fail(FAP_E_ARG, "boundary rows: ld_rb < V");
+  if ((int64_t)g->h_st2ed.size() != g->V)
+    return fail(FAP_E_STATE, "graph has no partition (fap_graph_set_partition)");
+  for (int32_t p = 0; p < count; p++) {
+    FAP_TRY(check_part(g, starts[p], n_subs[p], n_bdys[p]));
+    if (n_bdys[p] > 0 && !rowblocks[p]) return fail(FAP_E_ARG, "boundary rows: null row block");
+  }
+  return run_boundary_rows(ctx, g, count, starts, n_subs, n_bdys, rowblocks, ld_rb,
+                           pick_stream(ctx, stream));
 }
 
 int fap_subgraph_solve_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,

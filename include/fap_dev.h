@@ -231,6 +231,18 @@ int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count
                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                              int32_t *const *rowblocks, int64_t ld_rb, void *stream);
 
+/* Stage 1 alone for `count` parts (the reference's handle_boundry_path,
+ * fap/kernel/batch_sssp.h:254-277, called once per part by run(),
+ * fap/fap.cpp:105-110): rows 0..n_bdys[p]-1 of each part's row block
+ * (ld_rb >= V) get the distances from the part's boundary vertices to every
+ * vertex -- through the graph's boundary index (built by the first call
+ * when the context's "sssp_index" is on), else one SSSP launch over all the
+ * parts' boundary vertices.  The other rows are left as they are.
+ * Synchronous. */
+int fap_subgraph_boundary_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
+                               const int32_t *starts, const int32_t *n_subs,
+                               const int32_t *n_bdys, int32_t *const *rowblocks, int64_t ld_rb,
+                               void *stream);
 /* fap_subgraph_solve_batch over groups of `group_parts` parts, alternating
  * between the context's stream and a second context it creates on first
  * use: one group's SSSP (HBM-bound) runs beside the previous group's

@@ -492,6 +492,35 @@ def test_boundary_index_rows_equal_sssp_rows(gpu_ctx, n, K, wl, wh):
     dg.close()
 
 
+@pytest.mark.parametrize("index", [1, 0])
+def test_subgraph_boundary_rows_match_dijkstra(gpu_ctx, index):
+    """Stage 1 alone (fap_subgraph_boundary_rows, the reference's
+    handle_boundry_path for several parts): rows 0..b-1 of each part's row
+    block equal Dijkstra from the part's boundary vertices, through the
+    boundary index and through SSSP; the other rows are left alone."""
+    ro, col, wt, gid, pc = _rgg_partitioned(1100, 7, 91, 0, 40)
+    n = len(ro) - 1
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    ids = [2, 3, 6]
+    gpu_ctx.set_option("sssp_index", index)
+    try:
+        outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), n)) for p in ids]
+        for o in outs:
+            gpu_ctx.fill(o, -5)
+        dg.boundary_rows([int(pc["ver_off"][p]) for p in ids], [int(pc["ver_num"][p]) for p in ids],
+                         [int(pc["bdy_num"][p]) for p in ids], [o.ptr for o in outs], n)
+        for p, o in zip(ids, outs):
+            b, s0 = int(pc["bdy_num"][p]), int(pc["ver_off"][p])
+            got = o.get()
+            np.testing.assert_array_equal(got[:b], O.batched_sssp(pc["st2ed"][s0:s0 + b], ro, col, wt))
+            assert (got[b:] == -5).all()
+            o.free()
+    finally:
+        gpu_ctx.set_option("sssp_index", 1)
+    dg.close()
+
+
 def test_boundary_index_survives_staging_growth_between_calls():
     """Regression: a batched solve whose descriptor staging outgrows the
     previous call's (40 parts, ~12,000 sources, after a one-part call) must
