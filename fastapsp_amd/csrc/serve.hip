@@ -272,6 +272,11 @@ struct fap_server {
   uint64_t kick = 0;  // bumped by submit / release / stop
   bool stop = false;
   std::string fatal;
+  // test hook (environment FAP_SERVE_FAIL_MIXED_WAVE=n): the n-th mixed
+  // wave fails on the host after its big graphs were queued on the
+  // auxiliary stream, before the join -- the failure path the lane must
+  // survive without handing back blocks still being written
+  int fail_mixed_wave = 0;
   std::thread thr;
   fapi::Clock::time_point t0;
   fap_serve_stats st{};
@@ -442,6 +447,8 @@ int launch_wave(fap_server *S, Lane &L) {
       FAP_HIP(hipStreamWaitEvent(sb, L.fork, 0));
     }
     for (int32_t j : big) FAP_TRY(launch_fw(L.ctx, mats[j], ns[j], lds[j], sb));
+    if (split && S->fail_mixed_wave > 0 && --S->fail_mixed_wave == 0)
+      return fail(FAP_E_STATE, "injected failure of a mixed wave after the fork");
     if (!bmats.empty())
       FAP_TRY(launch_fw_batch(L.ctx, fd, (const fapk::BatchJob *)(L.d_stage + o_bj),
                               (const fapk::BatchSeg *)(L.d_stage + o_sg), s));
@@ -795,6 +802,7 @@ int fap_serve_create(int device, const fap_serve_opts *opts, fap_server **out) {
   S->device = device;
   S->opts = o;
   S->t0 = Clock::now();
+  if (const char *f = getenv("FAP_SERVE_FAIL_MIXED_WAVE")) S->fail_mixed_wave = atoi(f);
   S->lanes.resize(o.lanes);
   int rc = FAP_OK;
   hipError_t e = hipSetDevice(device);

@@ -90,6 +90,57 @@ def test_serve_large_graph_dense_path_matches_oracle():
     assert mixed, "no wave held both a big and a small graph"
 
 
+def test_serve_mixed_wave_failure_after_fork_leaves_the_lane_usable(monkeypatch):
+    """A mixed wave (a big graph on the lane's auxiliary stream, a small one
+    in the batched closure) that fails on the host after the fork -- forced
+    by the server's test hook, FAP_SERVE_FAIL_MIXED_WAVE -- fails both of
+    its requests with the error, the lane waits for both streams before the
+    wave's arena blocks go back, and later waves on the same lane (and the
+    same blocks) are exact."""
+    monkeypatch.setenv("FAP_SERVE_FAIL_MIXED_WAVE", "1")
+    big, small = rgg(3100, 51), rgg(200, 52)
+    blocker = rgg(4000, 53)
+    want_big, want_small = apsp(*big), apsp(*small)
+    failed = None
+    with fa.Server(hbm_budget=512 << 20, lanes=1) as srv:
+        for attempt in range(6):
+            b = srv.submit_graph(*blocker)          # occupies the lane for a few ms
+            ids = [srv.submit_graph(*big), srv.submit_graph(*small)]
+            errs = []
+            for rid, want in zip(ids, (want_big, want_small)):
+                try:
+                    srv.wait(rid)
+                    np.testing.assert_array_equal(srv.rows(rid), want)
+                    errs.append(None)
+                except fa.FapError as e:
+                    errs.append(str(e))
+                srv.release(rid)
+            nfail = 0
+            try:
+                srv.wait(b)
+            except fa.FapError as e:             # the blocker shared the failed wave
+                assert "injected failure of a mixed wave" in str(e)
+                nfail = 1
+            srv.release(b)
+            if any(errs):
+                failed = errs
+                nfail += sum(1 for e in errs if e)
+                break
+        if failed is not None:
+            # the injected failure hits the requests of that wave together
+            assert all(e and "injected failure of a mixed wave" in e for e in failed), failed
+            for _ in range(2):                       # the lane and its blocks still work
+                ids = [srv.submit_graph(*big), srv.submit_graph(*small)]
+                for rid, want in zip(ids, (want_big, want_small)):
+                    assert srv.wait(rid)["state"] == "done"
+                    np.testing.assert_array_equal(srv.rows(rid), want)
+                    srv.release(rid)
+            st = srv.stats()
+            assert st["failed"] == nfail and st["arena_used"] == 0
+    if failed is None:
+        pytest.skip("host timing never queued a big and a small graph in one wave")
+
+
 def test_serve_queries_match_oracle():
     ro, col, w = grid(40, 5)
     V = len(ro) - 1
