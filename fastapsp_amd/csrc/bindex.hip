@@ -108,6 +108,8 @@ struct fap_bidx {
   BidxPart *parts = nullptr;       // device, K
   int32_t *in_off = nullptr;       // nbp + 1: cut arcs into each H id
   int2 *in_arc = nullptr;          // {source H id, weight}
+  int2 *in1 = nullptr;             // nbp: {source H id, weight} of an H id's only cut arc,
+                                   // else {-1 - first arc, end of its arcs} (none: {-1, 0})
   int32_t *hloc = nullptr;         // nbp: local (st2ed) index of each H id's vertex
   // borders: part q's in-neighbour slots inb_off[q] .. inb_off[q + 1] - 1;
   // slot e's H positions of q (entered by cut arcs from that neighbour,
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
 template <int TM, int TN>
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN == 4 ? 5 : 4))) void bidx_block_kernel(
     const int32_t *__restrict__ ctrl, const BidxTile *__restrict__ tiles, int32_t *X, int64_t ldx,
-    const BidxPart *__restrict__ parts, const int32_t *__restrict__ in_off,
+    const BidxPart *__restrict__ parts, const int2 *__restrict__ in1,
     const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb,
     const int32_t *__restrict__ inb_off, const int32_t *__restrict__ inb_pos_off,
     const int32_t *__restrict__ inb_pos, int32_t *tmin) {
@@ -251,6 +253,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
   constexpr int TR = Tile::BM, TC = Tile::BN;  // rows, columns of a tile
   __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
   __shared__ int32_t klist[kBidxKlMax];
+  __shared__ int2 kinf[kBidxKlMax];  // in1 of each listed position
   const int ntiles = ctrl[BC_NTILES];
   const int t = threadIdx.x;
   const int arow = t / Tile::A_TPR, akq = (t % Tile::A_TPR) * TM;  // A staging (the tile's layout)
@@ -268,7 +271,11 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
       const int si = sl - s0;
       if (!((T.slots >> min(si, 31)) & 1u)) continue;
       const int p0 = inb_pos_off[sl], len = inb_pos_off[sl + 1] - p0;
-      for (int i = t; i < len; i += MP_THREADS) klist[kn + i] = inb_pos[p0 + i];
+      for (int i = t; i < len; i += MP_THREADS) {
+        const int pos = inb_pos[p0 + i];
+        klist[kn + i] = pos;
+        kinf[kn + i] = in1[Q.hoff + pos];
+      }
       kn += len;
     }
     const int K = kn;
@@ -285,11 +292,14 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
         const int k = k0 + akq + j;
         int v = kInf;
         if (arow < m_valid && k < K) {
-          const int h = Q.hoff + klist[k];
-          const int e1 = in_off[h + 1];
-          for (int e = in_off[h]; e < e1; e++) {
-            const int2 a = in_arc[e];
-            v = min(v, xa[a.x] + a.y);
+          const int2 f = kinf[k];
+          if (f.x >= 0) {
+            v = min(kInf, xa[f.x] + f.y);  // the position's only cut arc (the common case)
+          } else {
+            for (int e = -1 - f.x; e < f.y; e++) {
+              const int2 a = in_arc[e];
+              v = min(v, xa[a.x] + a.y);
+            }
           }
         }
         ra[j] = v;
@@ -455,7 +465,7 @@ namespace {
 
 void free_bidx(fap_bidx *x) {
   if (!x) return;
-  void *bufs[] = {x->parts,   x->in_off, x->in_arc, x->hloc,     x->inb_off, x->inb_pos_off,
+  void *bufs[] = {x->parts,   x->in_off, x->in_arc, x->in1,      x->hloc,    x->inb_off, x->inb_pos_off,
                   x->inb_pos, x->outb_off, x->outb,  x->lbb,      x->lbv,     x->colid,
                   x->ctiles};
   for (void *b : bufs)
@@ -621,7 +631,7 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   // cut arcs into each H id; the borders: for each ordered pair of parts
   // (q, q2) with a cut arc q -> q2, the H positions of q2 it enters
   std::vector<int32_t> in_off(x->nbp + 1, 0);
-  std::vector<int2> in_arc;
+  std::vector<int2> in_arc, in1;
   std::vector<int32_t> inb_off(K + 1, 0), inb_pos_off(1, 0), inb_pos, outb_off(K + 1, 0);
   std::vector<int2> outb;
   {
@@ -635,9 +645,11 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
         lists[hid[v]].push_back(make_int2(hid[u], ed[e].y));
         cuts[gid[v] - 1].push_back({gid[u] - 1, hid[v] - x->h_parts[gid[v] - 1].hoff});
       }
+    in1.assign(x->nbp, make_int2(-1, 0));
     for (int32_t h = 0; h < x->nbp; h++) {
       in_off[h + 1] = in_off[h] + (int32_t)lists[h].size();
       in_arc.insert(in_arc.end(), lists[h].begin(), lists[h].end());
+      in1[h] = lists[h].size() == 1 ? lists[h][0] : make_int2(-1 - in_off[h], in_off[h + 1]);
     }
     std::vector<std::vector<int2>> outs(K);  // part q: {q2, slot of q in q2}
     for (int32_t q2 = 0; q2 < K; q2++) {
@@ -675,6 +687,7 @@ int bidx_build(fap_ctx *ctx, fap_graph_dev *g) {
   if (rc == FAP_OK) rc = upload(x->h_parts, &x->parts, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(in_off, &x->in_off, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(in_arc, &x->in_arc, &x->bytes, s);
+  if (rc == FAP_OK) rc = upload(in1, &x->in1, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(hloc, &x->hloc, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(inb_off, &x->inb_off, &x->bytes, s);
   if (rc == FAP_OK) rc = upload(inb_pos_off, &x->inb_pos_off, &x->bytes, s);
@@ -846,12 +859,12 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
       if (tcol == 64)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 4>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
                            (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
-                           x->parts, x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
+                           x->parts, x->in1, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
                            x->inb_pos, (int32_t *)(d + o_tmin));
       else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 8>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
                            (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
-                           x->parts, x->in_off, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
+                           x->parts, x->in1, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
                            x->inb_pos, (int32_t *)(d + o_tmin));
       FAP_CHECK_LAUNCH();
     }
