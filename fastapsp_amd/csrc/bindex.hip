@@ -78,16 +78,16 @@ struct BidxPart {
 // tc*128.. of part q's boundary; the
 // product runs over the border positions of q entered from the parts
 // whose bits are set in `slots` (bit s: q's s-th in-neighbour part; bit 31
-// stands for every in-neighbour from the 31st on).  `task` indexes the
-// task's smallest-improvement word.
+// stands for every in-neighbour from the 31st on).  `blk` is the block's
+// index p * K + q in the pending array.
 struct BidxTile {
-  int32_t row0, rows, q, tr, tc, task;
+  int32_t row0, rows, q, tr, tc, blk;
   uint32_t slots;
   int32_t pad;
 };
 
 // Control words of the block iterations (device).
-enum { BC_THR = 0, BC_NTASKS = 1, BC_NTILES = 2, BC_DONE = 3, BC_WORDS = 16 };
+enum { BC_THR = 0, BC_NTILES = 2, BC_DONE = 3, BC_WORDS = 16 };
 
 // Longest border-position list a tile may take (the sum of a part's
 // in-border lengths); a graph with a longer one gets no index.
@@ -146,30 +146,41 @@ __global__ void bidx_block_init_kernel(int32_t *__restrict__ X, int64_t ldx, int
   }
 }
 
-// Scheduling step of the block iterations (one workgroup).  Folds the
-// previous launch's per-task smallest improvements into the pending blocks
-// (pend[p * K + q], INF = not pending), raises the band threshold past the
-// smallest pending value when none lies below it, takes every pending
-// block below the threshold (clearing it) and marks, for each part q2 a
-// cut arc from its part q enters, the slot of q among q2's in-neighbours in
-// mask[p * K + q2]; then lists the marked blocks as tasks and their
-// 64 x tcol tiles.  With nothing pending it sets the done word and lists nothing, so
-// launches enqueued after convergence are no-ops.
+// LDS copies the scheduling step keeps when they fit (global otherwise):
+// the marks of up to kSchedBlocks blocks, boundary sizes and out-neighbour
+// offsets of up to kSchedParts parts.
+constexpr int kSchedBlocks = 8192, kSchedParts = 2048;
+
+// Scheduling step of the block iterations (one workgroup).  pend[p * K + q]
+// is a block's smallest changed value (INF: not pending; the block launch
+// lowers it as it improves values).  The step raises the band threshold
+// past the smallest pending value when none lies below it, takes every
+// pending block below the threshold (clearing it), marks for each part q2
+// that a cut arc from the block's part q enters the slot of q among q2's
+// in-neighbours (mask[p * K + q2]), and lists the 64 x tcol tiles of the
+// marked blocks.  With nothing pending it sets the done word and lists
+// nothing, so launches enqueued after convergence are no-ops.
 __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
-    int32_t *ctrl, int32_t *pend, uint32_t *mask, int32_t *tasks, int32_t *tmin,
-    BidxTile *tiles, const int2 *__restrict__ prow, int32_t count, int32_t K, int32_t delta,
+    int32_t *ctrl, int32_t *pend, uint32_t *mask, BidxTile *tiles, const int2 *__restrict__ prow, int32_t count, int32_t K, int32_t delta,
     int32_t tcol, const BidxPart *__restrict__ parts, const int32_t *__restrict__ outb_off,
     const int2 *__restrict__ outb) {
   __shared__ int s_red[16];
-  __shared__ int s_thr, s_nt, s_ntl;
+  __shared__ int s_thr, s_ntl;
+  __shared__ uint32_t smask[kSchedBlocks];
+  __shared__ int32_t sb[kSchedParts];
+  __shared__ int32_t soff[kSchedParts + 1];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nblk = count * K;
-  const int prev = ctrl[BC_NTASKS];
-  for (int t = tid; t < prev; t += nthr) {
-    const int f = tmin[t];
-    if (f < kInf) atomicMin(&pend[tasks[t]], f);
+  const bool lmask = nblk <= kSchedBlocks, lparts = K <= kSchedParts;
+  if (lmask)
+    for (int i = tid; i < nblk; i += nthr) smask[i] = 0;
+  if (lparts) {
+    for (int q = tid; q < K; q += nthr) {
+      sb[q] = parts[q].b;
+      soff[q] = outb_off[q];
+    }
+    if (tid == 0) soff[K] = outb_off[K];
   }
-  __threadfence();
   __syncthreads();
   int lo = kInf;
   for (int i = tid; i < nblk; i += nthr) lo = min(lo, pend[i]);
@@ -185,49 +196,49 @@ __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
     s_thr = m < kInf ? thr : -1;
     ctrl[BC_THR] = thr;
     ctrl[BC_DONE] = m < kInf ? 0 : 1;
-    s_nt = 0;
     s_ntl = 0;
   }
   __syncthreads();
   const int thr = s_thr;
   if (thr < 0) {
-    if (tid == 0) {
-      ctrl[BC_NTASKS] = 0;
-      ctrl[BC_NTILES] = 0;
-    }
+    if (tid == 0) ctrl[BC_NTILES] = 0;
     return;
   }
   for (int i = tid; i < nblk; i += nthr) {
     if (pend[i] >= thr) continue;
     pend[i] = kInf;
     const int p = i / K, q = i - p * K;
-    for (int e = outb_off[q]; e < outb_off[q + 1]; e++) {
+    const int e0 = lparts ? soff[q] : outb_off[q], e1 = lparts ? soff[q + 1] : outb_off[q + 1];
+    for (int e = e0; e < e1; e++) {
       const int2 o = outb[e];
-      if (parts[o.x].b > 0) atomicOr(&mask[p * K + o.x], 1u << min(o.y, 31));
+      const int b = lparts ? sb[o.x] : parts[o.x].b;
+      if (b <= 0) continue;
+      const uint32_t bit = 1u << min(o.y, 31);
+      if (lmask) atomicOr(&smask[p * K + o.x], bit);
+      else atomicOr(&mask[p * K + o.x], bit);
     }
   }
-  __threadfence();
+  if (!lmask) __threadfence();
   __syncthreads();
   for (int i = tid; i < nblk; i += nthr) {
-    const uint32_t m = mask[i];
+    uint32_t m;
+    if (lmask) {
+      m = smask[i];
+    } else {
+      m = mask[i];
+      if (m) mask[i] = 0;
+    }
     if (!m) continue;
-    mask[i] = 0;
     const int p = i / K, q2 = i - p * K;
-    const int t = atomicAdd(&s_nt, 1);
-    tasks[t] = i;
-    tmin[t] = kInf;
     const int2 pr = prow[p];
-    const int rt = (pr.y + 63) >> 6, ct = (parts[q2].b + tcol - 1) / tcol;
+    const int rt = (pr.y + 63) >> 6, ct = ((lparts ? sb[q2] : parts[q2].b) + tcol - 1) / tcol;
     const int base = atomicAdd(&s_ntl, rt * ct);
     for (int tr = 0; tr < rt; tr++)
       for (int tc = 0; tc < ct; tc++)
-        tiles[base + tr * ct + tc] = BidxTile{pr.x, pr.y, q2, tr, tc, t, m, 0};
+        tiles[base + tr * ct + tc] = BidxTile{pr.x, pr.y, q2, tr, tc, i, m, 0};
   }
   __syncthreads();
-  if (tid == 0) {
-    ctrl[BC_NTASKS] = s_nt;
-    ctrl[BC_NTILES] = s_ntl;
-  }
+  if (tid == 0) ctrl[BC_NTILES] = s_ntl;
 }
 
 // Block updates of one iteration, the tiles listed by the scheduling step
@@ -236,7 +247,7 @@ __global__ __launch_bounds__(1024) void bidx_schedule_kernel(
 // (collected in LDS first); A (64 rows x k) is E, staged chunk by chunk
 // straight from the cut arcs into q (E[i][k] = min over arcs c' -> hoff_q +
 // k of X[i][c'] + w); B is the listed rows of L_q's boundary block; C is X,
-// kept where not improved.  The task's word in tmin takes the smallest
+// kept where not improved.  The block's pending word takes the smallest
 // improved value.  Positions left out are entered only from parts whose
 // blocks did not change since this block was last updated with them.  X
 // values read by the gather may be lowered concurrently by other tiles;
@@ -248,7 +259,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
     const BidxPart *__restrict__ parts, const int2 *__restrict__ in1,
     const int2 *__restrict__ in_arc, const int32_t *__restrict__ lbb,
     const int32_t *__restrict__ inb_off, const int32_t *__restrict__ inb_pos_off,
-    const int32_t *__restrict__ inb_pos, int32_t *tmin) {
+    const int32_t *__restrict__ inb_pos, int32_t *pend) {
   using Tile = MpTile<TM, TN>;
   constexpr int TR = Tile::BM, TC = Tile::BN;  // rows, columns of a tile
   __shared__ __attribute__((aligned(16))) int32_t lds[Tile::LDS_WORDS];
@@ -361,7 +372,7 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(TN =
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off));
-    if (lane == 0 && lowest < kInf) atomicMin(&tmin[T.task], lowest);
+    if (lane == 0 && lowest < kInf) atomicMin(&pend[T.blk], lowest);
     __syncthreads();  // klist and the staging are rewritten by the next tile
   }
 }
@@ -792,7 +803,7 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
   FAP_TRY(ensure_ws(ctx, ctx->ws_bidx, (size_t)nsrc * ldx * sizeof(int32_t)));
   int32_t *X = (int32_t *)ctx->ws_bidx.ptr;
   // workspace: control | block rows {first X row, rows} | source rows {part,
-  // H position} | pending | marks | tasks | task minima | tiles
+  // H position} | pending | marks | tiles
   // block-update tile: 64 rows x tcol columns (tcol = 64 or 128)
   const int tcol = ctx->opt_bidx_tile == 64 ? 64 : 128;
   int64_t rt_sum = 0, ct_sum = 0;
@@ -804,12 +815,10 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
   const size_t o_rowq = o_prow + up(sizeof(int2) * count);
   const size_t o_pend = o_rowq + up(sizeof(int2) * nsrc);
   const size_t o_mask = o_pend + up(sizeof(int32_t) * nblk);
-  const size_t o_task = o_mask + up(sizeof(uint32_t) * nblk);
-  const size_t o_tmin = o_task + up(sizeof(int32_t) * nblk);
-  const size_t o_tile = o_tmin + up(sizeof(int32_t) * nblk);
+  const size_t o_tile = o_mask + up(sizeof(uint32_t) * nblk);
   const size_t total = o_tile + sizeof(BidxTile) * (size_t)max_tiles;
   char *h = nullptr, *d = nullptr;
-  FAP_TRY(bbf_buffers(ctx, total, o_task, &h, &d));
+  FAP_TRY(bbf_buffers(ctx, total, o_tile, &h, &d));
   int32_t *hc = (int32_t *)h;
   std::fill(hc, hc + BC_WORDS, 0);
   // band: the graph's (mean arc weight x sqrt(V / K)) for 32 or more parts
@@ -831,7 +840,7 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
     for (int32_t i = 0; i < b; i++) hrowq[rowoff[p] + i] = make_int2(q, x->h_hpos[x->boff[q] + i]);
     if (b > 0) hpend[(size_t)p * K + q] = 0;  // each block starts as its own part's distances
   }
-  FAP_HIP(hipMemcpyAsync(d, h, o_task, hipMemcpyHostToDevice, s));
+  FAP_HIP(hipMemcpyAsync(d, h, o_tile, hipMemcpyHostToDevice, s));
   int32_t *dctrl = (int32_t *)d;
   hipLaunchKernelGGL(fapk::bidx_block_init_kernel, dim3(nsrc), dim3(256), 0, s, X, ldx, x->nbp,
                      (const int2 *)(d + o_rowq), x->parts, x->lbb);
@@ -851,21 +860,19 @@ int bidx_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t
     if (it > ((int64_t)1 << 22)) return fail(FAP_E_STATE, "boundary iterations did not settle");
     for (int i = 0; i < kBatch; i++) {
       hipLaunchKernelGGL(fapk::bidx_schedule_kernel, dim3(1), dim3(1024), 0, s, dctrl,
-                         (int32_t *)(d + o_pend), (uint32_t *)(d + o_mask),
-                         (int32_t *)(d + o_task), (int32_t *)(d + o_tmin),
-                         (BidxTile *)(d + o_tile), (const int2 *)(d + o_prow), count, K, delta, tcol,
+                         (int32_t *)(d + o_pend), (uint32_t *)(d + o_mask), (BidxTile *)(d + o_tile), (const int2 *)(d + o_prow), count, K, delta, tcol,
                          x->parts, x->outb_off, x->outb);
       FAP_CHECK_LAUNCH();
       if (tcol == 64)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 4>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
                            (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
                            x->parts, x->in1, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
-                           x->inb_pos, (int32_t *)(d + o_tmin));
+                           x->inb_pos, (int32_t *)(d + o_pend));
       else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(fapk::bidx_block_kernel<4, 8>), dim3(grid), dim3(fapk::MP_THREADS), 0, s,
                            (const int32_t *)dctrl, (const BidxTile *)(d + o_tile), X, ldx,
                            x->parts, x->in1, x->in_arc, x->lbb, x->inb_off, x->inb_pos_off,
-                           x->inb_pos, (int32_t *)(d + o_tmin));
+                           x->inb_pos, (int32_t *)(d + o_pend));
       FAP_CHECK_LAUNCH();
     }
     FAP_HIP(hipMemcpyAsync(h, d, sizeof(int32_t) * BC_WORDS, hipMemcpyDeviceToHost, s));
