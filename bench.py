@@ -491,6 +491,7 @@ def grid_section(args, grid, modes=("plain", "pipelined")):
     for b in rbs:
         b.free()
     sssp["index_ms_per_call"] = round(idx_ms, 3)
+    sssp["index_edges_per_s_equiv"] = round(S * E / (idx_ms * 1e-3), 1)  # same rows, S.E / t
     sssp["index_over_sssp"] = round(sssp_ms / idx_ms, 2)
     log(f"[bench] c3 SSSP: {sssp}")
     P = max(G, min(args.call_parts, len(parts)))
