@@ -521,6 +521,40 @@ def test_subgraph_boundary_rows_match_dijkstra(gpu_ctx, index):
     dg.close()
 
 
+@pytest.mark.parametrize("R,K,ids", [(64, 96, None), (150, 2100, (1, 700, 1400, 2100))])
+def test_boundary_index_scheduling_beyond_lds_tables(gpu_ctx, R, K, ids):
+    """The scheduling step keeps block marks in LDS up to 8,192 blocks (parts
+    in the call x K) and part tables up to 2,048 parts; beyond, it works from
+    global memory.  96 parts in one call (9,216 blocks) and a 2,100-part
+    graph (both beyond): every part's rows equal those of the SSSP stage and
+    of the oracle."""
+    V, ro, col, wt, gid, pc = partitioned_grid(R, K, R + K)
+    dg = gpu_ctx.upload_graph(ro, col, wt)
+    dg.set_partition(gid, pc["st2ed"], pc["ed2st"])
+    assert dg.build_index()
+    ids = list(ids) if ids else list(range(1, K + 1))
+    sums = {}
+    for flag in (1, 0):
+        gpu_ctx.set_option("sssp_index", flag)
+        try:
+            outs = [gpu_ctx.alloc((int(pc["ver_num"][p]), V)) for p in ids]
+            dg.subgraph_solve_batch([int(pc["ver_off"][p]) for p in ids],
+                                    [int(pc["ver_num"][p]) for p in ids],
+                                    [int(pc["bdy_num"][p]) for p in ids], [o.ptr for o in outs], V)
+            sums[flag] = [gpu_ctx.checksum(o.ptr, int(pc["ver_num"][p]) * V)
+                          for o, p in zip(outs, ids)]
+            if flag == 1:
+                for p, o in list(zip(ids, outs))[:: max(1, len(ids) // 3)]:
+                    np.testing.assert_array_equal(o.get(), O.run_part(p, ro, col, wt, gid, pc),
+                                                  err_msg=f"part {p}")
+            for o in outs:
+                o.free()
+        finally:
+            gpu_ctx.set_option("sssp_index", 1)
+    assert sums[1] == sums[0]
+    dg.close()
+
+
 def test_boundary_index_survives_staging_growth_between_calls():
     """Regression: a batched solve whose descriptor staging outgrows the
     previous call's (40 parts, ~12,000 sources, after a one-part call) must
