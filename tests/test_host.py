@@ -353,3 +353,56 @@ def test_host_code_under_address_and_undefined_sanitizers(tmp_path):
                        env=env)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout + r.stderr)[-4000:]
     assert "runtime error" not in r.stderr, r.stderr[-4000:]
+
+
+def test_boundary_index_host_code_with_emulated_kernels_under_sanitizers(tmp_path):
+    """The boundary index's host code (fastapsp_amd/csrc/bindex.hip: index
+    build, border order and lists, the block iterations' workspace and
+    bookkeeping), compiled as it is against a host-only stand-in of the HIP
+    runtime and CPU emulations of its kernels (tests/cpp/bidx_host), under
+    -fsanitize=address,undefined: a 64 x 64 grid cut in 16 parts, boundary
+    rows for calls of 4 and of all 16 parts, sampled rows equal to Dijkstra."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("no g++")
+    R, K = 64, 16
+    V = R * R
+    u, v, w = synth.grid_graph(R, R, 0.10, 0.02, 1, 100, 3)
+    ro, col, wt = synth.to_csr(V, u, v, w, False)
+    with fa.Graph.from_csr(ro, col, wt.astype(np.float32), K) as g:
+        g.preCondition()
+        vn, vo, bn, _ = g.part_info()
+        st2ed, gid = g.getMapping(), g.getGraphId()
+    ed2st = np.empty(V, np.int32)
+    ed2st[st2ed] = np.arange(V, dtype=np.int32)
+    gpath = str(tmp_path / "grid.bin")
+    with open(gpath, "wb") as f:
+        np.array([V, len(col), K], np.int64).tofile(f)
+        for a in (ro, col, wt, gid, st2ed, ed2st, vn, vo, bn):
+            np.asarray(a, np.int32).tofile(f)
+    src = open(os.path.join(ROOT, "fastapsp_amd", "csrc", "bindex.hip")).read()
+    a = src.index("struct BidxPart {")
+    b = src.index("namespace fapk {")
+    c = src.index("namespace fapi {\n\nnamespace {\n\nvoid free_bidx")
+    (tmp_path / "bindex_host.inc").write_text(src[a:b] + '\n#include "emu_kernels.h"\n' + src[c:])
+    here = os.path.join(ROOT, "tests", "cpp", "bidx_host")
+    inc = ["-I" + os.path.join(here, "mock"), "-I" + here, "-I" + str(tmp_path),
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "fastapsp_amd", "csrc")]
+    fw_o = str(tmp_path / "fw_emu.o")
+    r = subprocess.run([gxx, "-std=c++17", "-O2", "-c", os.path.join(here, "fw_emu.cpp"), "-o", fw_o],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    exe = str(tmp_path / "bidx_host")
+    r = subprocess.run([gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                        "-fno-sanitize-recover=undefined"] + inc +
+                       [os.path.join(here, "harness.cpp"), fw_o, "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    for args in (["4"], ["16", "6"]):     # calls of 4 parts; one call of 16 (6 row buffers)
+        r = subprocess.run([exe, gpath] + args, capture_output=True, text=True, timeout=600,
+                           env=env)
+        assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
+        assert "0 bad" in r.stdout and "runtime error" not in r.stderr, r.stdout + r.stderr
