@@ -158,19 +158,16 @@ int run_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start, int3
   return FAP_OK;
 }
 
-// All parts of a batch together: one SSSP launch for every boundary
-// vertex of every part (rows written straight into each part's row block),
-// the part matrices gathered side by side and closed by one batched closure,
-// then the per-part min-plus products and scatters.  Synchronous.
-// Enqueues the batched solve of `count` parts on stream s without waiting:
-// one SSSP launch for every boundary vertex (a row pointer per source into
-// its part's row block), the part matrices gathered side by side in ws_sub,
-// one batched closure, then each part's product and scatter.  Sources, row
-// pointers and closure descriptors go up in one copy from the context's
-// pinned staging.  ctx's workspaces stay in use until s passes this work;
-// a later enqueue on the same stream is ordered after it.  The SSSP launch
-// waits for `sssp_after` (if given) and is followed by a record of
-// `sssp_done` (if given).
+// Enqueues the batched solve of `count` parts on stream s: every boundary
+// vertex's row (a row pointer per source into its part's row block) through
+// the boundary index when it holds every part -- the host waits on s while
+// the index's iterations run -- else one SSSP launch; the part matrices
+// gathered side by side in ws_sub, one batched closure, then each part's
+// product and scatter.  Sources, row pointers and closure descriptors go up
+// in one copy from the context's pinned staging.  ctx's workspaces stay in
+// use until s passes this work; a later enqueue on the same stream is
+// ordered after it.  Stage 1 waits for `sssp_after` (if given) and is
+// followed by a record of `sssp_done` (if given).
 int enqueue_subgraph_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
                            const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                            int32_t *const *rowblocks, int64_t ld_rb, hipStream_t s,
@@ -310,11 +307,11 @@ int run_boundary_rows(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const
 }
 
 // Groups of `group_parts` parts, alternating between ctx's stream and its
-// peer context's: one group's SSSP (HBM-bound) runs beside the other's
-// closure and products (VALU-bound).  Each group's SSSP waits for the
-// previous group's SSSP: two SSSP launches side by side took longer than
-// one after the other (2,048 concurrent sources thrash the caches).
-// Synchronous.
+// peer context's: one group's stage 1 (boundary index iterations, or SSSP)
+// runs beside the other's closure and products (VALU-bound).  Each group's
+// stage 1 waits for the previous group's: two SSSP launches side by side
+// took longer than one after the other (2,048 concurrent sources thrash the
+// caches).  Synchronous.
 int run_subgraph_groups(fap_ctx *ctx, const fap_graph_dev *g, int32_t count, const int32_t *starts,
                         const int32_t *n_subs, const int32_t *n_bdys, int32_t *const *rowblocks,
                         int64_t ld_rb, int32_t group_parts) {

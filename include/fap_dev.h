@@ -224,9 +224,11 @@ int fap_subgraph_solve(fap_ctx *ctx, const fap_graph_dev *g, int32_t start,
 
 /* Batched form of fap_subgraph_solve over `count` parts (host arrays of
  * starts, sizes, boundary counts and device row-block pointers, stride
- * ld_rb): one SSSP launch for all boundary vertices, one batched closure of
- * all part matrices, then per-part min-plus and scatter.  Same rows as
- * fap_subgraph_solve on each part.  Synchronous. */
+ * ld_rb): every part's boundary rows through the graph's boundary index
+ * (built by the first such call when "sssp_index" is on; one SSSP launch
+ * for all boundary vertices when it is off or the index declines), one
+ * batched closure of all part matrices, then per-part min-plus and scatter.
+ * Same rows as fap_subgraph_solve on each part.  Synchronous. */
 int fap_subgraph_solve_batch(fap_ctx *ctx, const fap_graph_dev *g, int32_t count,
                              const int32_t *starts, const int32_t *n_subs, const int32_t *n_bdys,
                              int32_t *const *rowblocks, int64_t ld_rb, void *stream);
