@@ -196,7 +196,35 @@ struct MpTile {
       }
     }
 
-    // epilogue
+    // epilogue.  Accumulating into a full tile, the C pieces of RG rows are
+    // all loaded before any of them is stored: stores to C may alias later
+    // loads as far as the compiler knows (the row stride is a run-time
+    // value), so loads interleaved with stores would each wait for the
+    // previous piece's round trip to memory.
+    if (accumulate && m_valid == BM && n_valid == BN && (flags & MP_VEC_C)) {
+      constexpr int HN = TN / 4;            // 16-byte pieces per row and lane
+      constexpr int RG = TM >= 8 ? 2 : 4;   // rows per round trip
+#pragma unroll
+      for (int r0 = 0; r0 < TM; r0 += RG) {
+        int4 o[RG][HN];
+#pragma unroll
+        for (int rr = 0; rr < RG; rr++)
+#pragma unroll
+          for (int h = 0; h < HN; h++)
+            o[rr][h] = *reinterpret_cast<const int4 *>(C + (int64_t)(row0 + 8 * (r0 + rr)) * ldc +
+                                                       col0 + 32 * h);
+#pragma unroll
+        for (int rr = 0; rr < RG; rr++)
+#pragma unroll
+          for (int h = 0; h < HN; h++) {
+            const int r = r0 + rr;
+            *reinterpret_cast<int4 *>(C + (int64_t)(row0 + 8 * r) * ldc + col0 + 32 * h) =
+                make_int4(imin(acc[r][4 * h + 0], o[rr][h].x), imin(acc[r][4 * h + 1], o[rr][h].y),
+                          imin(acc[r][4 * h + 2], o[rr][h].z), imin(acc[r][4 * h + 3], o[rr][h].w));
+          }
+      }
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < TM; r++) {
       const int row = row0 + 8 * r;
