@@ -894,13 +894,17 @@ def batch_section(args, ctx, lib, ceiling):
     empty -- dense build + blocked FW).  Plain = one graph per launch
     sequence, looped on the host (fap/fap.cpp:189-193); batched = one
     phase-synchronous launch sequence over all graphs (size-ordered task
-    queues, paired pivot rounds, two chains), replayed from a hipGraph.
-    Checksums of all matrices must agree."""
+    queues, paired pivot rounds, two chains), replayed from a hipGraph;
+    persistent = every phase of every graph in ONE launch whose workgroups
+    claim tasks in queue order and wait in the kernel for their graph's
+    earlier phases (fap_fw_batched_persistent, synchronous, descriptors
+    built per call).  Rows are 128-byte aligned (ld a multiple of 32), as
+    the single launch requires.  Checksums of all matrices must agree."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
     sizes = synth.loguniform_sizes(args.jobs, 64, 2048, 4).astype(int)
     graphs, ptrs, lds = [], [], []
-    ld_of = [(n + 3) // 4 * 4 for n in sizes]
+    ld_of = [(n + 31) // 32 * 32 for n in sizes]
     total = sum(n * ld for n, ld in zip(sizes, ld_of))
     pool = ctx.alloc(total)
     off = 0
@@ -928,8 +932,14 @@ def batch_section(args, ctx, lib, ceiling):
     def batched():
         plan.run(ctx.stream)
 
+    ns = [int(x) for x in sizes]
+
+    def persistent():
+        ctx.fw_batched_persistent(ptrs, ns, lds, ctx.stream)
+
     res = {}
-    runs = (("batched", batched),) if args.batched_only else (("plain", plain), ("batched", batched))
+    runs = (("batched", batched),) if args.batched_only else (
+        ("plain", plain), ("batched", batched), ("persistent", persistent))
     for name, fn in runs:
         build()
         fn()
@@ -949,8 +959,10 @@ def batch_section(args, ctx, lib, ceiling):
     for g in graphs:
         g.close()
     pool.free()
-    same = None if args.batched_only else res["plain_sum"] == res["batched_sum"]
+    same = None if args.batched_only else (res["plain_sum"] == res["batched_sum"] ==
+                                           res["persistent_sum"])
     plain_ms = res.get("plain")
+    pers_ms = res.get("persistent")
     bt = res["batched"]
     return {"workload": f"{args.jobs} random geometric graphs, n log-uniform in [64, 2048], "
                         f"deg 6, directed, w 1..255; full APSP per graph as one part (K=1: "
@@ -964,6 +976,9 @@ def batch_section(args, ctx, lib, ceiling):
             "batched_frac_of_ceiling": round(ops / (bt * 1e-3) / 1e12 / ceiling, 3),
             "batched_frac_of_spec_peak": round(ops / (bt * 1e-3) / 1e12 / VALU_PEAK_TOPS, 3),
             "batched_launches": info["launches"], "batched_tasks": info["tasks"],
+            "persistent_ms": None if pers_ms is None else round(pers_ms, 3),
+            "persistent_GOp_s": None if pers_ms is None else round(ops / (pers_ms * 1e-3) / 1e9, 1),
+            "persistent_over_plain": None if pers_ms is None else round(plain_ms / pers_ms, 2),
             "steps": args.batch_steps, "identical": same, "tuned": tuned}
 
 

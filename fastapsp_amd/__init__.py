@@ -89,6 +89,8 @@ _SIGS = {
     "fap_fw_blocked": (_int, [_vp, _vp, _i32, _i64, _vp]),
     "fap_fw_batched": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _vp]),
     "fap_fw_batched_noalloc": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _vp]),
+    "fap_fw_batched_persistent": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64),
+                                         _vp]),
     "fap_fw_plan_create": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _int,
                                   C.POINTER(_vp)]),
     "fap_fw_plan_run": (_int, [_vp, _vp]),
@@ -439,6 +441,16 @@ class Context:
         L = np.ascontiguousarray(lds, dtype=np.int64)
         check(load().fap_fw_batched(self.handle, k, P, _p32(N), L.ctypes.data_as(C.POINTER(_i64)),
                                     stream), "fap_fw_batched")
+
+    def fw_batched_persistent(self, ptrs, ns, lds, stream=None):
+        """The batched closure as one launch (fap_fw_batched_persistent)."""
+        k = len(ptrs)
+        P = (_vp * k)(*ptrs)
+        N = np.ascontiguousarray(ns, dtype=np.int32)
+        L = np.ascontiguousarray(lds, dtype=np.int64)
+        check(load().fap_fw_batched_persistent(self.handle, k, P, _p32(N),
+                                               L.ctypes.data_as(C.POINTER(_i64)), stream),
+              "fap_fw_batched_persistent")
 
     def fw_plan(self, ptrs, ns, lds, use_graph=True):
         k = len(ptrs)

@@ -24,9 +24,13 @@
 //  * the launch sequence of a batch is a plan: descriptors uploaded once,
 //    launches captured in a hipGraph and replayed.
 //
-// There is no wait inside a kernel: every workgroup runs its tasks and
-// exits.  (A single-launch variant whose workgroups waited on per-job
-// completion counters made no progress on the device; it was taken out.)
+// The phase launches have no wait inside a kernel: every workgroup runs
+// its tasks and exits.  fw_queue_kernel runs the same tasks in ONE launch,
+// with bounded in-kernel waits on per-job completion counters; its
+// workgroups claim tasks in queue order, which is what lets every wait end
+// (an earlier single-launch variant dealt tasks statically, so resident
+// workgroups could wait on tasks of workgroups not yet dispatched, and it
+// made no progress on the device).
 #include <string.h>
 
 #include <algorithm>
@@ -74,6 +78,61 @@ __global__ __launch_bounds__(1024) void fw_diag_batch_kernel(const BatchJob *__r
   }
 }
 
+// The segment holding task ti: segments tile [0, ntasks) in order.
+template <typename Seg>
+__device__ __forceinline__ int find_seg(const Seg *__restrict__ segs, int nseg, int ti) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].first <= ti) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Task idx of a panel, single-pivot (rest and cross kinds) or pair-pass
+// phase with pivot block r of `job`: one 64 x 64 tile update.
+__device__ __forceinline__ void batch_tile_task(const BatchJob &job, int kind, int r, int idx,
+                                                int32_t *lds) {
+  const int nb = job.nb, n = job.n;
+  const int64_t ld = job.ld;
+  int32_t *d = job.mat;
+  const int base = r * BT;
+  const int kb = imin(BT, n - base);
+  const int flags =
+      ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
+  if (kind == TASK_PANEL) {
+    const int32_t *diag = d + (int64_t)base * ld + base;
+    int I, J;
+    panel_tile(r, nb, idx, &I, &J);
+    if (I == r) {
+      int32_t *c = d + (int64_t)base * ld + (int64_t)J * BT;
+      MpSmall::run(diag, ld, c, ld, c, ld, kb, imin(BT, n - J * BT), kb, true, flags, lds);
+    } else {
+      int32_t *c = d + (int64_t)I * BT * ld + base;
+      MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
+    }
+  } else if (kind == TASK_REST2) {
+    // pivots r and r + 1 at once: blocks r, r + 1 are adjacent, so the two
+    // A tiles of row I form one 64 x 128 panel and the two B tiles of
+    // column J one 128 x 64 panel
+    int I, J;
+    pair_tile(r, nb, idx, &I, &J);
+    MpSmall::run(d + (int64_t)I * BT * ld + base, ld, d + (int64_t)base * ld + (int64_t)J * BT,
+                 ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
+                 imin(BT, n - J * BT), imin(2 * BT, n - base), true, flags, lds);
+  } else {
+    // one tile: off row/column r with pivot r, or of row/column q with
+    // pivot p (cross kinds)
+    int I, J, p;
+    single_tile(kind, r, nb, idx, &I, &J, &p);
+    const int pb = p * BT;
+    MpSmall::run(d + (int64_t)I * BT * ld + pb, ld, d + (int64_t)pb * ld + (int64_t)J * BT,
+                 ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
+                 imin(BT, n - J * BT), imin(BT, n - pb), true, flags, lds);
+  }
+}
+
 // Panel, single-pivot (rest and cross kinds) and pair-pass tasks; every
 // task of a launch runs on one instance (KIND = kind_instance of its kind:
 // the phase patterns keep launches homogeneous and the host checks), so
@@ -88,56 +147,120 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
   static_assert(KIND == TASK_PANEL || KIND == TASK_REST || KIND == TASK_REST2, "kernel instance");
   __shared__ __attribute__((aligned(16))) int32_t lds[MpSmall::LDS_WORDS];
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
-    // the segment holding task ti: segments tile [0, ntasks) in order
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (segs[mid].first <= ti) lo = mid;
-      else hi = mid - 1;
-    }
-    const BatchSeg sg = segs[lo];
-    const BatchJob job = jobs[sg.job];
+    const BatchSeg sg = segs[find_seg(segs, nseg, ti)];
     const int kind = KIND == TASK_REST ? (sg.kind_r & 7) : KIND;
-    const int r = sg.kind_r >> 3, idx = ti - sg.first;
-    const int nb = job.nb, n = job.n;
-    const int64_t ld = job.ld;
-    int32_t *d = job.mat;
-    const int base = r * BT;
-    const int kb = imin(BT, n - base);
-    const int flags =
-        ((((uintptr_t)d) & 15) == 0 && (ld & 3) == 0) ? (MP_VEC_A | MP_VEC_B | MP_VEC_C) : 0;
-    if (KIND == TASK_PANEL) {
-      const int32_t *diag = d + (int64_t)base * ld + base;
-      int I, J;
-      panel_tile(r, nb, idx, &I, &J);
-      if (I == r) {
-        int32_t *c = d + (int64_t)base * ld + (int64_t)J * BT;
-        MpSmall::run(diag, ld, c, ld, c, ld, kb, imin(BT, n - J * BT), kb, true, flags, lds);
-      } else {
-        int32_t *c = d + (int64_t)I * BT * ld + base;
-        MpSmall::run(c, ld, diag, ld, c, ld, imin(BT, n - I * BT), kb, kb, true, flags, lds);
-      }
-    } else if (KIND == TASK_REST) {
-      // one tile: off row/column r with pivot r, or of row/column q with
-      // pivot p (cross kinds)
-      int I, J, p;
-      single_tile(kind, r, nb, idx, &I, &J, &p);
-      const int pb = p * BT;
-      MpSmall::run(d + (int64_t)I * BT * ld + pb, ld, d + (int64_t)pb * ld + (int64_t)J * BT,
-                   ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
-                   imin(BT, n - J * BT), imin(BT, n - pb), true, flags, lds);
-    } else {
-      // pivots r and r + 1 at once: blocks r, r + 1 are adjacent, so the two
-      // A tiles of row I form one 64 x 128 panel and the two B tiles of
-      // column J one 128 x 64 panel
-      int I, J;
-      pair_tile(r, nb, idx, &I, &J);
-      MpSmall::run(d + (int64_t)I * BT * ld + base, ld, d + (int64_t)base * ld + (int64_t)J * BT,
-                   ld, d + (int64_t)I * BT * ld + (int64_t)J * BT, ld, imin(BT, n - I * BT),
-                   imin(BT, n - J * BT), imin(2 * BT, n - base), true, flags, lds);
-    }
+    batch_tile_task(jobs[sg.job], kind, sg.kind_r >> 3, ti - sg.first, lds);
     __syncthreads();  // the LDS buffer is reused by the next task
   }
+}
+
+// XCD (accelerator complex die) the calling wave runs on: the XCC_ID
+// hardware register.  Each XCD has its own L2.
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
+
+// Bits of the XCDs a grid's workgroups run on.
+__global__ void xcc_probe_kernel(int32_t *mask) {
+  if (threadIdx.x == 0) atomicOr(mask, 1 << xcc_id());
+}
+
+// Every phase of every job in ONE launch.  The jobs are split over the
+// device's XCDs, one queue per XCD: a queue is the phase-ordered launch
+// sequence of its jobs laid end to end, and a workgroup serves the queue
+// of the XCD it runs on (qmap[xcc_id()]), claiming tasks in queue order
+// from that queue's counter, each claim made while the workgroup runs (the
+// next one while its current task computes).  A task first waits until
+// `need` tasks of its job have completed (done[job]): all of them come
+// earlier in the same queue, so each was claimed by a workgroup that is
+// running and waits only for tasks earlier still -- the first unfinished
+// task of a queue can always run, whatever the grid and however many
+// workgroups are resident.  The wait is bounded: past `limit` wall-clock
+// ticks, or once another workgroup has given up (ctrl[0] != 0), the
+// workgroup raises ctrl[0] and exits, and so does every other, so the
+// launch always ends; the host reports the flag, and checks every job's
+// count (a queue whose XCD ran no workgroup would be left unfinished).
+// Every task of a job runs on one XCD, so writer and reader share that
+// XCD's L2: a task's stores are complete in L2 before its count is raised
+// (s_waitcnt vmcnt(0) in every wave, then the barrier), with no L2
+// write-back; the next task's thread 0 takes an agent-scope acquire after
+// its wait (dropping stale lines of the CU's L1; an L1-only invalidate,
+// buffer_inv sc0, left stale lines and wrong results), and the barrier
+// orders every wave's loads after it.  Measured on the 512-graph batch:
+// agent-scope release and acquire in every wave 158 ms, XCD-local release
+// with every wave acquiring 38 ms, thread 0 acquiring 23 ms.  Diagonal
+// tasks close their block with 256 threads (close_block64).  All task
+// kinds in one kernel take 80 VGPRs (64 spill): 6 waves per SIMD.
+//   qinfo: [0, 16) qmap (XCC id -> queue or -1); then per queue q, at
+//          16 + 4q: first segment, segment count, task count
+//   ctrl:  [0] flag; claim counter of queue q at 32 (q + 1) (own cache line)
+__global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fw_queue_kernel(
+    const BatchJob *__restrict__ jobs, const BatchQSeg *__restrict__ all_segs,
+    const int32_t *__restrict__ qinfo, int32_t *ctrl, int32_t *done, int64_t limit) {
+  constexpr int W = BLK_LDS_WORDS > MpSmall::LDS_WORDS ? BLK_LDS_WORDS : MpSmall::LDS_WORDS;
+  __shared__ __attribute__((aligned(16))) int32_t lds[W];
+  __shared__ int32_t s_next, s_go;
+  const int t = threadIdx.x;
+  const int q = qinfo[xcc_id()];
+  if (q < 0) return;
+  const BatchQSeg *segs = all_segs + qinfo[16 + 4 * q];
+  const int nseg = qinfo[16 + 4 * q + 1], ntasks = qinfo[16 + 4 * q + 2];
+  int32_t *claim = ctrl + 32 * (q + 1);
+  if (t == 0) s_next = atomicAdd(claim, 1);
+  __syncthreads();
+  int ti = s_next;
+  while (ti < ntasks) {
+    const BatchQSeg sg = segs[find_seg(segs, nseg, ti)];
+    if (t == 0) {
+      int go = 1;
+      if (sg.need > 0) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(&done[sg.job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               sg.need) {
+          if (__hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+              (int64_t)(__builtin_amdgcn_s_memrealtime() - t0) > limit) {
+            go = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      if (go && __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        go = 0;
+      if (!go) {
+        __hip_atomic_store(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_next = ntasks;
+      } else {
+        s_next = atomicAdd(claim, 1);  // the next claim, while this task runs
+        // the CU's L1 may hold lines of this job's tiles from before their
+        // last writes: drop them (an agent-scope acquire; the barrier below
+        // orders every wave's loads after it)
+        if (sg.need > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      s_go = go;
+    }
+    __syncthreads();
+    const int next = s_next, go = s_go;  // read before thread 0 can write them again
+    if (!go) break;
+    const BatchJob job = jobs[sg.job];
+    const int kind = sg.kind_r & 7, r = sg.kind_r >> 3;
+    if (kind == TASK_DIAG) {
+      const int base = r * BT;
+      close_block64(job.mat + (int64_t)base * job.ld + base, job.ld, imin(BT, job.n - base), lds);
+    } else {
+      batch_tile_task(job, kind, r, ti - sg.first, lds);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete in L2
+    __syncthreads();                // ... every wave's; the LDS buffer is free again
+    if (t == 0) __hip_atomic_fetch_add(&done[sg.job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ti = next;
+  }
+}
+
+// Clears a queue launch's flag, claim counters and per-job counters (a
+// kernel, not a memset, so the sequence can be captured).
+__global__ void fw_queue_reset_kernel(int32_t *ctrl, int nctrl, int32_t *done, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nctrl) ctrl[i] = 0;
+  if (i < count) done[i] = 0;
 }
 
 }  // namespace fapk
@@ -203,6 +326,25 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
     out->seg_off.push_back((int32_t)out->segs.size());
   }
   out->key = fap_tune_key_batch(count, n);
+}
+
+int32_t build_fw_queue(const FwBatchDesc &desc, std::vector<fapk::BatchQSeg> *out) {
+  out->clear();
+  std::vector<int32_t> before(desc.jobs.size(), 0);  // tasks of each job queued so far
+  int32_t first = 0;
+  for (size_t L = 0; L < desc.launch_tasks.size(); L++) {
+    // a launch holds at most one phase per job: its tasks need all of the
+    // job's tasks queued before this launch
+    for (int q = desc.seg_off[L]; q < desc.seg_off[L + 1]; q++) {
+      const fapk::BatchSeg &sg = desc.segs[q];
+      out->push_back(fapk::BatchQSeg{sg.job, sg.kind_r, first + sg.first, sg.count,
+                                     before[sg.job], 0});
+    }
+    for (int q = desc.seg_off[L]; q < desc.seg_off[L + 1]; q++)
+      before[desc.segs[q].job] += desc.segs[q].count;
+    first += desc.launch_tasks[L];
+  }
+  return first;
 }
 
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
@@ -453,6 +595,157 @@ static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, c
 
 using namespace fapi;
 
+// XCDs of the context's device (bit mask), probed once: a grid of a few
+// workgroups per CU records the XCC ids it runs on.
+static int device_xcc_mask(fap_ctx *ctx, hipStream_t s, int *mask) {
+  if (ctx->xcc_mask <= 0) {
+    int32_t *d = nullptr, h = 0;
+    FAP_HIP(hipMalloc(&d, 256));
+    hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3(1), dim3(64), 0, s, d, 1,
+                       (int32_t *)nullptr, 0);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(fapk::xcc_probe_kernel, dim3(std::max(ctx->num_cus, 1) * 4), dim3(64), 0,
+                         s, d);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "XCD probe", __FILE__, __LINE__);
+    if (h <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: XCD probe saw no XCC id");
+    ctx->xcc_mask = h;
+  }
+  *mask = ctx->xcc_mask;
+  return FAP_OK;
+}
+
+// The batch in one launch (fw_queue_kernel): jobs split over the device's
+// XCDs by work (2 n^3, largest first, each to the least loaded), one queue
+// per XCD; then the flag and every job's count.
+static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                        const int64_t *ld, hipStream_t s) {
+  // tasks of one phase write distinct tiles, and a tile's rows must not
+  // share a 128-byte cache line with another tile's
+  for (int32_t j = 0; j < count; j++)
+    if (n[j] > 0 && ((((uintptr_t)d_mats[j]) & 127) != 0 || (ld[j] & 31) != 0))
+      return fail(FAP_E_ARG, "fw_batched_persistent: matrix " + std::to_string(j) +
+                                 " needs 128-byte aligned rows (base address a multiple of 128, "
+                                 "ld a multiple of 32)");
+  FAP_HIP(hipSetDevice(ctx->device));
+  int khz = 0;
+  FAP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: no wall clock rate");
+  const int64_t wait_us = ctx->opt_batch_wait_us > 0 ? ctx->opt_batch_wait_us : 2000000;
+  const int64_t limit = wait_us * khz / 1000;
+  int mask = 0;
+  FAP_TRY(device_xcc_mask(ctx, s, &mask));
+  std::vector<int> xccs;
+  for (int x = 0; x < 16; x++)
+    if (mask & (1 << x)) xccs.push_back(x);
+  const int nq = (int)std::min<size_t>(xccs.size(), (size_t)count);
+
+  // jobs to queues
+  std::vector<int32_t> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  std::vector<double> load(nq, 0.0);
+  std::vector<std::vector<int32_t>> qjobs(nq);
+  for (int j : order) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    qjobs[q].push_back(j);
+    load[q] += 2.0 * n[j] * (double)n[j] * n[j] + 1.0;
+  }
+  // per queue: its own launch sequence laid end to end; jobs and segments
+  // concatenated over the queues (segment job indices made global)
+  std::vector<fapk::BatchJob> jobs;
+  std::vector<fapk::BatchQSeg> segs;
+  std::vector<int32_t> qinfo(16 + 4 * nq, 0), job_total;
+  for (int x = 0; x < 16; x++) qinfo[x] = -1;
+  int64_t ntasks = 0;
+  for (int q = 0; q < nq; q++) {
+    qinfo[xccs[q]] = q;
+    std::vector<int32_t *> m;
+    std::vector<int32_t> nn;
+    std::vector<int64_t> ll;
+    for (int j : qjobs[q]) {
+      m.push_back(d_mats[j]);
+      nn.push_back(n[j]);
+      ll.push_back(ld[j]);
+    }
+    FwBatchDesc d;
+    build_fw_batch((int32_t)m.size(), m.data(), nn.data(), ll.data(), &d);
+    std::vector<fapk::BatchQSeg> qs;
+    const int32_t qt = build_fw_queue(d, &qs);
+    if ((int64_t)qt + (int64_t)ctx->num_cus * 8 >= INT32_MAX)
+      return fail(FAP_E_ARG, "fw_batched_persistent: too many tasks");
+    const int32_t job_base = (int32_t)jobs.size();
+    qinfo[16 + 4 * q] = (int32_t)segs.size();
+    qinfo[16 + 4 * q + 1] = (int32_t)qs.size();
+    qinfo[16 + 4 * q + 2] = qt;
+    for (const fapk::BatchJob &jb : d.jobs) {
+      jobs.push_back(jb);
+      job_total.push_back(0);
+    }
+    for (fapk::BatchQSeg g : qs) {
+      g.job += job_base;
+      job_total[g.job] += g.count;
+      segs.push_back(g);
+    }
+    ntasks += qt;
+  }
+  if (ntasks == 0) return FAP_OK;
+  const int njobs = (int)jobs.size();
+  const size_t jobs_b = sizeof(fapk::BatchJob) * jobs.size();
+  const size_t seg_off = (jobs_b + 255) & ~(size_t)255;
+  const size_t qi_off = (seg_off + sizeof(fapk::BatchQSeg) * segs.size() + 255) & ~(size_t)255;
+  const size_t ctrl_off = (qi_off + sizeof(int32_t) * qinfo.size() + 255) & ~(size_t)255;
+  const int nctrl = 32 * (nq + 1);
+  const size_t done_off = ctrl_off + sizeof(int32_t) * nctrl;
+  const size_t total = done_off + sizeof(int32_t) * njobs;
+  std::vector<char> host(ctrl_off, 0);
+  memcpy(host.data(), jobs.data(), jobs_b);
+  memcpy(host.data() + seg_off, segs.data(), sizeof(fapk::BatchQSeg) * segs.size());
+  memcpy(host.data() + qi_off, qinfo.data(), sizeof(int32_t) * qinfo.size());
+  char *dev = nullptr;
+  FAP_HIP(hipMalloc(&dev, total));
+  int32_t *ctrl = (int32_t *)(dev + ctrl_off), *done = (int32_t *)(dev + done_off);
+  std::vector<int32_t> got(njobs + 1, 0);
+  hipError_t e = hipMemcpyAsync(dev, host.data(), ctrl_off, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3((std::max(nctrl, njobs) + 255) / 256),
+                       dim3(256), 0, s, ctrl, nctrl, done, njobs);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    // 6 resident workgroups per CU; workgroups are dealt to the XCDs in turn
+    const int grid = ctx->num_cus * 6;
+    hipLaunchKernelGGL(fapk::fw_queue_kernel, dim3(grid), dim3(fapk::MP_THREADS), 0, s,
+                       (const fapk::BatchJob *)dev, (const fapk::BatchQSeg *)(dev + seg_off),
+                       (const int32_t *)(dev + qi_off), ctrl, done, limit);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(got.data(), done, sizeof(int32_t) * njobs,
+                                          hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(got.data() + njobs, ctrl, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "fw_batched_persistent", __FILE__, __LINE__);
+  if (got[njobs] != 0)
+    return fail(FAP_E_DEVICE, "fw_batched_persistent: a task waited longer than " +
+                                  std::to_string((long long)wait_us) +
+                                  " us for its job's earlier phases; the launch gave up "
+                                  "(matrices partly closed)");
+  for (int j = 0; j < njobs; j++)
+    if (got[j] != job_total[j])
+      return fail(FAP_E_DEVICE, "fw_batched_persistent: a matrix's queue ran " +
+                                    std::to_string(got[j]) + " of its " +
+                                    std::to_string(job_total[j]) +
+                                    " tasks (no workgroup on its XCD)");
+  return FAP_OK;
+}
+
 extern "C" {
 
 int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
@@ -500,6 +793,14 @@ int fap_fw_batched_noalloc(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, 
                           (const fapk::BatchSeg *)(scratch + segs_off), s));
   FAP_HIP(hipStreamSynchronize(s));
   return FAP_OK;
+}
+
+int fap_fw_batched_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mats,
+                              const int32_t *n, const int64_t *ld, void *stream) {
+  if (!ctx) return fail(FAP_E_ARG, "ctx is null");
+  FAP_TRY(check_jobs(count, d_mats, n, ld));
+  if (count == 0) return FAP_OK;
+  return run_fw_queue(ctx, count, d_mats, n, ld, pick_stream(ctx, stream));
 }
 
 int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,

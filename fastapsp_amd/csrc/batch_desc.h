@@ -34,6 +34,19 @@ struct BatchSeg {
   int32_t count;
 };
 
+// One phase of one job in the single-launch queue (fw_queue_kernel): tasks
+// [first, first + count) of the whole queue, which start only once `need`
+// tasks of the same job have completed -- every task of the job's earlier
+// phases, all of which come earlier in the queue.
+struct BatchQSeg {
+  int32_t job;
+  int32_t kind_r;
+  int32_t first;
+  int32_t count;
+  int32_t need;
+  int32_t pad;
+};
+
 // Phases of a job's blocked Floyd-Warshall.  Rounds go in pairs (r, r + 1):
 //   DIAG(r), PANEL(r), CROSS_A(r), DIAG(r+1), PANEL(r+1), REST2(r), CROSS_B(r)
 // where CROSS_A updates row and column r+1 with pivot r (what DIAG and PANEL
@@ -143,6 +156,9 @@ struct FwBatchDesc {
 
 void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const int64_t *ld,
                     FwBatchDesc *out);
+// The launches of `desc` as one queue, launch after launch (so a task's
+// own job's earlier phases always come before it); returns the task count.
+int32_t build_fw_queue(const FwBatchDesc &desc, std::vector<fapk::BatchQSeg> *out);
 // Launch sequence of a batch whose jobs and segments are already on the
 // device (same order as in `desc`).
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,

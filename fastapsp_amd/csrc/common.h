@@ -74,6 +74,7 @@ struct fap_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int xcc_mask = 0;  // XCDs the device's workgroups run on (bits of XCC ids); 0 = not probed
   size_t lds_per_block = 0;
   std::mutex mu;  // serialises workspace growth
   // named workspaces (device)
@@ -110,6 +111,7 @@ struct fap_ctx {
   int opt_sssp_index = 1;        // batched part solves: stage 1 through the boundary index (1) or SSSP (0)
   int opt_bidx_tile = 0;         // boundary iterations' tile columns: 64 or 128; 0 = 128
   int opt_bidx_delta = 0;        // distance band of the boundary iterations; 0 = mean arc weight x sqrt(V / K)
+  int64_t opt_batch_wait_us = 0;  // single-launch batched closure: longest wait of a task; 0 = 2 s
   fap_tune_cache *tune = nullptr;  // not owned
   // pinned host staging for asynchronous descriptor uploads; free for reuse
   // once ev_stage (recorded after the last upload from it) has completed

@@ -263,6 +263,10 @@ int fap_ctx_set_option(fap_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "bidx_delta") {
     if (value < 0 || value > FAP_INF / 4) return fail(FAP_E_ARG, "bidx_delta out of range");
     ctx->opt_bidx_delta = (int)value;
+  } else if (k == "batch_wait_us") {
+    if (value < 0 || value > 600000000LL)
+      return fail(FAP_E_ARG, "batch_wait_us must be 0 (auto) .. 600000000");
+    ctx->opt_batch_wait_us = value;
   } else if (k == "batch_grid") {
     if (value < 0 || value > 64) return fail(FAP_E_ARG, "batch_grid must be 0 (auto) .. 64");
     ctx->opt_batch_grid = (int)value;
@@ -288,6 +292,7 @@ int fap_ctx_get_option(fap_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_index") *value = ctx->opt_sssp_index;
   else if (k == "bidx_delta") *value = ctx->opt_bidx_delta;
   else if (k == "bidx_tile") *value = ctx->opt_bidx_tile;
+  else if (k == "batch_wait_us") *value = ctx->opt_batch_wait_us;
   else return fail(FAP_E_ARG, "unknown option '" + k + "'");
   return FAP_OK;
 }

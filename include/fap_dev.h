@@ -86,6 +86,8 @@ int fap_ctx_stream(fap_ctx *ctx, void **stream);
  *   "sssp_wgs"     0..8     SSSP workgroups per CU [1024 threads per CU]
  *   "sssp_arcs"    0/1/2/4  arcs each lane relaxes per SSSP step, loads issued together [2]
  *   "batch_grid"   0..64    batched-closure workgroups per CU per launch [64]
+ *   "batch_wait_us" >= 0    longest in-kernel wait of a fap_fw_batched_persistent task
+ *                           for its matrix's earlier phases, in microseconds [2 s]
  *   "sssp_index"   0/1      batched part solves compute stage 1 through the graph's
  *                           boundary index (built by the first such call) [1]
  *   "bidx_tile"    0/64/128 columns of the boundary index's 64-row block-update tiles [128]
@@ -169,6 +171,24 @@ int fap_fw_batched(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const in
 #define FAP_NOALLOC_SCRATCH_BYTES (4 << 20)
 int fap_fw_batched_noalloc(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                            const int64_t *ld, void *stream);
+
+/* fap_fw_batched as ONE launch: the matrices are split over the device's
+ * XCDs by work, each XCD's matrices go into one queue (their phase launches
+ * laid end to end, larger matrices first within each), and the workgroups
+ * on each XCD claim that queue's tasks in order from a counter; a task
+ * waits, inside the kernel, until its own matrix's earlier phases have
+ * completed.  Replaces the reference's host loop over parts
+ * (fap/fap.cpp:189-193) and its per-process parts (fap/fap_more_task_mpi.cpp:34-91)
+ * with one launch; same results as fap_fw_batched.  Every wait is bounded
+ * (option "batch_wait_us"): a wait that runs out stops the launch and the
+ * call returns FAP_E_DEVICE with the matrices partly closed (also if a
+ * queue's XCD ran no workgroup).  Rows must be 128-byte aligned (each
+ * matrix's address a multiple of 128 bytes, ld a multiple of 32), else
+ * FAP_E_ARG: matrices on different XCDs then never share a cache line.
+ * Slower than the plans' phase launches on the 512-graph batch (23 vs
+ * 17 ms).  Synchronous. */
+int fap_fw_batched_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mats,
+                              const int32_t *n, const int64_t *ld, void *stream);
 
 /* A reusable batch: descriptors uploaded once; with use_graph the phase
  * launches are captured in a hipGraph and each run replays it (the

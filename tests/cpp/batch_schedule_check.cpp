@@ -6,9 +6,12 @@
 // the launch's one kernel instance; at the end every matrix equals plain
 // Floyd-Warshall (fap/kernel/floyd.h:38-49, k-i-j order) bit for bit.
 //
-// Second part: a model of ONE persistent launch running the same tasks, in
-// which a workgroup waits (spinning on per-job counters) for its job's
-// previous phase.  The hardware dispatches a grid's workgroups in order of
+// Second part: a model of ONE persistent launch running the same tasks as
+// the library queues them for fw_queue_kernel (fapi::build_fw_queue: the
+// launches laid end to end, each segment with the number of its job's tasks
+// that must complete first, checked against the launch sequence), in which
+// a workgroup waits (spinning on its job's completion counter) for its
+// job's earlier phases.  The hardware dispatches a grid's workgroups in order of
 // their id as slots free up and never preempts a resident one, so at most
 // R workgroups make progress at a time.  The model replays exactly that
 // (R slots, in-order dispatch, one task step per running workgroup per
@@ -22,7 +25,10 @@
 //  * tasks claimed in list order from a shared counter (a workgroup claims
 //    its next task only when it is running): drains for any G and R, since
 //    the first unfinished task is always claimed, by a running workgroup,
-//    and everything it waits for comes earlier in the list;
+//    and everything it waits for comes earlier in the list; the same when,
+//    as in the kernel, the next task is claimed once the current task's
+//    wait is over (the holder of the first unfinished task is then running
+//    it, or has finished everything before it);
 //  * every wait bounded by a spin limit that, when exceeded, raises an
 //    error flag and exits the workgroup: the launch always ends, and the
 //    host sees the flag.
@@ -177,33 +183,46 @@ static bool check_batch(const std::vector<int> &sizes, uint32_t seed) {
 
 struct PTask {
   int job, kind, r, idx;
-  int phase;      // index of this task's launch among its job's launches
+  int need;  // tasks of its job that must have completed first
 };
 
-struct PList {
-  std::vector<PTask> tasks;                 // phase-ordered (launch by launch)
-  std::vector<std::vector<int>> phase_size;  // [job][phase] task count
-};
-
-static PList persistent_list(const fapi::FwBatchDesc &desc, int count) {
-  PList pl;
-  pl.phase_size.assign(count, {});
+// The single-launch queue exactly as the library builds it
+// (fapi::build_fw_queue), checked against the launch sequence: segments
+// tile [0, T) in launch order, and each one's `need` is the number of its
+// job's tasks in earlier launches.
+static bool queue_list(const fapi::FwBatchDesc &desc, int count, std::vector<PTask> *out) {
+  std::vector<fapk::BatchQSeg> q;
+  const int32_t T = fapi::build_fw_queue(desc, &q);
+  std::vector<int> before(count, 0);
+  size_t qi = 0;
+  int32_t first = 0;
   for (size_t L = 0; L < desc.launch_tasks.size(); L++) {
-    std::vector<int> seen(count, 0);
-    for (int q = desc.seg_off[L]; q < desc.seg_off[L + 1]; q++) {
-      const fapk::BatchSeg &sg = desc.segs[q];
-      const int job = sg.job;
-      if (!seen[job]) {
-        seen[job] = 1;
-        pl.phase_size[job].push_back(0);
+    for (int s = desc.seg_off[L]; s < desc.seg_off[L + 1]; s++, qi++) {
+      const fapk::BatchSeg &sg = desc.segs[s];
+      if (qi >= q.size()) {
+        printf("queue: %zu segments, launches hold more\n", q.size());
+        return false;
       }
-      const int ph = (int)pl.phase_size[job].size() - 1;
-      for (int idx = 0; idx < sg.count; idx++)
-        pl.tasks.push_back(PTask{job, sg.kind_r & 7, sg.kind_r >> 3, idx, ph});
-      pl.phase_size[job][ph] += sg.count;
+      const fapk::BatchQSeg &g = q[qi];
+      if (g.job != sg.job || g.kind_r != sg.kind_r || g.count != sg.count ||
+          g.first != first || g.need != before[sg.job]) {
+        printf("queue segment %zu: job %d kind_r %d first %d count %d need %d; expected job %d "
+               "kind_r %d first %d count %d need %d\n", qi, g.job, g.kind_r, g.first, g.count,
+               g.need, sg.job, sg.kind_r, first, sg.count, before[sg.job]);
+        return false;
+      }
+      for (int idx = 0; idx < g.count; idx++)
+        out->push_back(PTask{g.job, g.kind_r & 7, g.kind_r >> 3, idx, g.need});
+      first += g.count;
     }
+    for (int s = desc.seg_off[L]; s < desc.seg_off[L + 1]; s++)
+      before[desc.segs[s].job] += desc.segs[s].count;
   }
-  return pl;
+  if (qi != q.size() || first != T) {
+    printf("queue: %zu segments / %d tasks, launches give %zu / %d\n", q.size(), T, qi, first);
+    return false;
+  }
+  return true;
 }
 
 static void run_task(std::vector<Mat> &m, const fapi::FwBatchDesc &desc, const PTask &t) {
@@ -224,61 +243,60 @@ static void run_task(std::vector<Mat> &m, const fapi::FwBatchDesc &desc, const P
   }
 }
 
-enum class Deal { Static, Claim };
+// Static: workgroup b takes tasks b, b + G, ... (the removed design).
+// Claim: a running workgroup claims its next task from a shared counter
+// when it has finished the current one.  ClaimAhead: as fw_queue_kernel,
+// it claims the next one as soon as the current task's wait is over.
+enum class Deal { Static, Claim, ClaimAhead };
 enum class Outcome { Drained, Stuck, GaveUp };
 
 // G workgroups, at most R resident; spin_limit < 0: unbounded waits.
-static Outcome simulate(const PList &pl, const fapi::FwBatchDesc &desc, std::vector<Mat> &m,
-                        int G, int R, Deal deal, long spin_limit) {
-  const int T = (int)pl.tasks.size();
-  std::vector<std::vector<int>> done(pl.phase_size.size());
-  for (size_t j = 0; j < done.size(); j++) done[j].assign(pl.phase_size[j].size(), 0);
-  auto ready = [&](const PTask &t) {
-    return t.phase == 0 || done[t.job][t.phase - 1] == pl.phase_size[t.job][t.phase - 1];
-  };
-  struct WG { int task = -1; long spins = 0; };
+static Outcome simulate(const std::vector<PTask> &tasks, int count,
+                        const fapi::FwBatchDesc &desc, std::vector<Mat> &m, int G, int R,
+                        Deal deal, long spin_limit) {
+  const int T = (int)tasks.size();
+  std::vector<int> done(count, 0);
+  auto ready = [&](const PTask &t) { return done[t.job] >= t.need; };
+  struct WG { int b = 0, task = -1, ahead = -1; long spins = 0; };
   std::vector<WG> resident;
   int next_wg = 0, claim = 0, finished = 0;
   bool flag = false;
-  // next task of workgroup b after task `cur` (-1: its first); T: none left
-  auto next_task = [&](int b, int cur) {
+  auto take = [&](int b, int cur) {
     if (deal == Deal::Static) {
       const long t = cur < 0 ? b : (long)cur + G;
       return t < T ? (int)t : T;
     }
     return claim < T ? claim++ : T;
   };
-  std::vector<int> wg_id;
   while (finished < T || !resident.empty()) {
     while ((int)resident.size() < R && next_wg < G) {  // in-order dispatch into free slots
       WG w;
-      w.task = next_task(next_wg, -1);
+      w.b = next_wg++;
+      w.task = take(w.b, -1);
       resident.push_back(w);
-      wg_id.push_back(next_wg);
-      next_wg++;
     }
     bool progress = false;
     for (size_t i = 0; i < resident.size();) {
       WG &w = resident[i];
-      if (w.task < T && ready(pl.tasks[w.task])) {
-        run_task(m, desc, pl.tasks[w.task]);
-        done[pl.tasks[w.task].job][pl.tasks[w.task].phase]++;
+      if (w.task < T && ready(tasks[w.task])) {
+        if (deal == Deal::ClaimAhead) w.ahead = take(w.b, w.task);  // claimed before it runs
+        run_task(m, desc, tasks[w.task]);
+        done[tasks[w.task].job]++;
         finished++;
-        w.task = next_task(wg_id[i], w.task);
+        w.task = deal == Deal::ClaimAhead ? w.ahead : take(w.b, w.task);
         w.spins = 0;
         progress = true;
       } else if (w.task < T) {
         if (spin_limit >= 0) {
           progress = true;  // a bounded wait runs toward its limit
-          if (++w.spins > spin_limit) {
-            flag = true;  // gives up: error flag, workgroup exits
+          if (++w.spins > spin_limit || flag) {
+            flag = true;  // gives up (or sees the flag): raises it, exits
             w.task = T;
           }
         }
       }
       if (w.task >= T) {  // exits, freeing its slot
         resident.erase(resident.begin() + i);
-        wg_id.erase(wg_id.begin() + i);
         progress = true;
         continue;
       }
@@ -302,11 +320,12 @@ static bool check_persistent(const std::vector<int> &sizes, uint32_t seed) {
   std::vector<int64_t> ld(sizes.begin(), sizes.end());
   fapi::FwBatchDesc desc;
   fapi::build_fw_batch(count, ptrs.data(), n.data(), ld.data(), &desc);
-  const PList pl = persistent_list(desc, count);
-  const int T = (int)pl.tasks.size();
+  std::vector<PTask> tasks;
+  if (!queue_list(desc, count, &tasks)) return false;
+  const int T = (int)tasks.size();
   auto expect = [&](const char *what, int G, int R, Deal d, long lim, Outcome want_o) {
     std::vector<Mat> m = mats;
-    const Outcome o = simulate(pl, desc, m, G, R, d, lim);
+    const Outcome o = simulate(tasks, count, desc, m, G, R, d, lim);
     if (o != want_o) {
       printf("persistent model, %s (G = %d, R = %d, %d tasks): outcome %d, expected %d\n", what,
              G, R, T, (int)o, (int)want_o);
@@ -328,9 +347,15 @@ static bool check_persistent(const std::vector<int> &sizes, uint32_t seed) {
     return false;
   // every workgroup resident
   if (!expect("static deal, G <= R", R, R, Deal::Static, -1, Outcome::Drained)) return false;
-  // in-order claims: drains whatever the grid
-  for (int G : {1, R, 4 * R, 64 * R})
+  // in-order claims, also one claim ahead (fw_queue_kernel): drain whatever the grid
+  for (int G : {1, R, 4 * R, 64 * R}) {
     if (!expect("claimed in order", G, R, Deal::Claim, -1, Outcome::Drained)) return false;
+    if (!expect("claimed in order, one ahead", G, R, Deal::ClaimAhead, -1, Outcome::Drained))
+      return false;
+    if (!expect("claimed one ahead, bounded wait", G, R, Deal::ClaimAhead, 1000000,
+                Outcome::Drained))
+      return false;
+  }
   return true;
 }
 
