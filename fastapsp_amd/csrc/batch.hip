@@ -190,11 +190,14 @@ __global__ void xcc_probe_kernel(int32_t *mask) {
 // tasks close their block with 256 threads (close_block64).  All task
 // kinds in one kernel take 80 VGPRs (64 spill): 6 waves per SIMD.
 //   qinfo: [0, 16) qmap (XCC id -> queue or -1); then per queue q, at
-//          16 + 4q: first segment, segment count, task count
+//          16 + 4q: first segment, segment count, task count, first entry
+//          of its task -> segment map
+//   segmap: task -> segment (within its queue), filled by fw_queue_map_kernel
 //   ctrl:  [0] flag; claim counter of queue q at 32 (q + 1) (own cache line)
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fw_queue_kernel(
     const BatchJob *__restrict__ jobs, const BatchQSeg *__restrict__ all_segs,
-    const int32_t *__restrict__ qinfo, int32_t *ctrl, int32_t *done, int64_t limit) {
+    const int32_t *__restrict__ qinfo, const int32_t *__restrict__ all_segmap, int32_t *ctrl,
+    int32_t *done, int64_t limit) {
   constexpr int W = BLK_LDS_WORDS > MpSmall::LDS_WORDS ? BLK_LDS_WORDS : MpSmall::LDS_WORDS;
   __shared__ __attribute__((aligned(16))) int32_t lds[W];
   __shared__ int32_t s_next, s_go;
@@ -202,13 +205,14 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
   const int q = qinfo[xcc_id()];
   if (q < 0) return;
   const BatchQSeg *segs = all_segs + qinfo[16 + 4 * q];
-  const int nseg = qinfo[16 + 4 * q + 1], ntasks = qinfo[16 + 4 * q + 2];
+  const int ntasks = qinfo[16 + 4 * q + 2];
+  const int32_t *segmap = all_segmap + qinfo[16 + 4 * q + 3];
   int32_t *claim = ctrl + 32 * (q + 1);
   if (t == 0) s_next = atomicAdd(claim, 1);
   __syncthreads();
   int ti = s_next;
   while (ti < ntasks) {
-    const BatchQSeg sg = segs[find_seg(segs, nseg, ti)];
+    const BatchQSeg sg = segs[segmap[ti]];
     if (t == 0) {
       int go = 1;
       if (sg.need > 0) {
@@ -252,6 +256,19 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
     __syncthreads();                // ... every wave's; the LDS buffer is free again
     if (t == 0) __hip_atomic_fetch_add(&done[sg.job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ti = next;
+  }
+}
+
+// Task -> segment map of every queue: workgroup b writes segment b's
+// entries (segments of all queues, each queue's map at its own offset).
+__global__ void fw_queue_map_kernel(const BatchQSeg *__restrict__ all_segs,
+                                    const int32_t *__restrict__ seg_map_off,
+                                    const int32_t *__restrict__ seg_local, int nseg,
+                                    int32_t *segmap) {
+  for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
+    const BatchQSeg sg = all_segs[b];
+    int32_t *m = segmap + seg_map_off[b] + sg.first;
+    for (int i = threadIdx.x; i < sg.count; i += blockDim.x) m[i] = seg_local[b];
   }
 }
 
@@ -660,7 +677,7 @@ static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, con
   // concatenated over the queues (segment job indices made global)
   std::vector<fapk::BatchJob> jobs;
   std::vector<fapk::BatchQSeg> segs;
-  std::vector<int32_t> qinfo(16 + 4 * nq, 0), job_total;
+  std::vector<int32_t> qinfo(16 + 4 * nq, 0), job_total, seg_map_off, seg_local;
   for (int x = 0; x < 16; x++) qinfo[x] = -1;
   int64_t ntasks = 0;
   for (int q = 0; q < nq; q++) {
@@ -677,12 +694,13 @@ static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, con
     build_fw_batch((int32_t)m.size(), m.data(), nn.data(), ll.data(), &d);
     std::vector<fapk::BatchQSeg> qs;
     const int32_t qt = build_fw_queue(d, &qs);
-    if ((int64_t)qt + (int64_t)ctx->num_cus * 8 >= INT32_MAX)
+    if (ntasks + qt + (int64_t)ctx->num_cus * 8 >= INT32_MAX)
       return fail(FAP_E_ARG, "fw_batched_persistent: too many tasks");
     const int32_t job_base = (int32_t)jobs.size();
     qinfo[16 + 4 * q] = (int32_t)segs.size();
     qinfo[16 + 4 * q + 1] = (int32_t)qs.size();
     qinfo[16 + 4 * q + 2] = qt;
+    qinfo[16 + 4 * q + 3] = (int32_t)ntasks;
     for (const fapk::BatchJob &jb : d.jobs) {
       jobs.push_back(jb);
       job_total.push_back(0);
@@ -690,6 +708,8 @@ static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, con
     for (fapk::BatchQSeg g : qs) {
       g.job += job_base;
       job_total[g.job] += g.count;
+      seg_map_off.push_back((int32_t)ntasks);
+      seg_local.push_back((int32_t)(segs.size() - qinfo[16 + 4 * q]));
       segs.push_back(g);
     }
     ntasks += qt;
@@ -699,14 +719,19 @@ static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, con
   const size_t jobs_b = sizeof(fapk::BatchJob) * jobs.size();
   const size_t seg_off = (jobs_b + 255) & ~(size_t)255;
   const size_t qi_off = (seg_off + sizeof(fapk::BatchQSeg) * segs.size() + 255) & ~(size_t)255;
-  const size_t ctrl_off = (qi_off + sizeof(int32_t) * qinfo.size() + 255) & ~(size_t)255;
+  const size_t smo_off = (qi_off + sizeof(int32_t) * qinfo.size() + 255) & ~(size_t)255;
+  const size_t sl_off = (smo_off + sizeof(int32_t) * segs.size() + 255) & ~(size_t)255;
+  const size_t ctrl_off = (sl_off + sizeof(int32_t) * segs.size() + 255) & ~(size_t)255;
   const int nctrl = 32 * (nq + 1);
   const size_t done_off = ctrl_off + sizeof(int32_t) * nctrl;
-  const size_t total = done_off + sizeof(int32_t) * njobs;
+  const size_t map_off = (done_off + sizeof(int32_t) * njobs + 255) & ~(size_t)255;
+  const size_t total = map_off + sizeof(int32_t) * (size_t)ntasks;
   std::vector<char> host(ctrl_off, 0);
   memcpy(host.data(), jobs.data(), jobs_b);
   memcpy(host.data() + seg_off, segs.data(), sizeof(fapk::BatchQSeg) * segs.size());
   memcpy(host.data() + qi_off, qinfo.data(), sizeof(int32_t) * qinfo.size());
+  memcpy(host.data() + smo_off, seg_map_off.data(), sizeof(int32_t) * segs.size());
+  memcpy(host.data() + sl_off, seg_local.data(), sizeof(int32_t) * segs.size());
   char *dev = nullptr;
   FAP_HIP(hipMalloc(&dev, total));
   int32_t *ctrl = (int32_t *)(dev + ctrl_off), *done = (int32_t *)(dev + done_off);
@@ -718,11 +743,19 @@ static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, con
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
+    hipLaunchKernelGGL(fapk::fw_queue_map_kernel, dim3(std::min<size_t>(segs.size(), 65535)),
+                       dim3(256), 0, s, (const fapk::BatchQSeg *)(dev + seg_off),
+                       (const int32_t *)(dev + smo_off), (const int32_t *)(dev + sl_off),
+                       (int)segs.size(), (int32_t *)(dev + map_off));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
     // 6 resident workgroups per CU; workgroups are dealt to the XCDs in turn
     const int grid = ctx->num_cus * 6;
     hipLaunchKernelGGL(fapk::fw_queue_kernel, dim3(grid), dim3(fapk::MP_THREADS), 0, s,
                        (const fapk::BatchJob *)dev, (const fapk::BatchQSeg *)(dev + seg_off),
-                       (const int32_t *)(dev + qi_off), ctrl, done, limit);
+                       (const int32_t *)(dev + qi_off), (const int32_t *)(dev + map_off), ctrl,
+                       done, limit);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(got.data(), done, sizeof(int32_t) * njobs,

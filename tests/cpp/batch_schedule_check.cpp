@@ -7,9 +7,9 @@
 // Floyd-Warshall (fap/kernel/floyd.h:38-49, k-i-j order) bit for bit.
 //
 // Second part: a model of ONE persistent launch running the same tasks as
-// the library queues them for fw_queue_kernel (fapi::build_fw_queue: the
-// launches laid end to end, each segment with the number of its job's tasks
-// that must complete first, checked against the launch sequence), in which
+// the library queues them for fw_queue_kernel (fapi::build_fw_queue: every
+// job's phases, launch after launch, each segment with the number of its job's tasks that must complete first,
+// checked against the launch sequence), in which
 // a workgroup waits (spinning on its job's completion counter) for its
 // job's earlier phases.  The hardware dispatches a grid's workgroups in order of
 // their id as slots free up and never preempts a resident one, so at most
@@ -188,38 +188,47 @@ struct PTask {
 
 // The single-launch queue exactly as the library builds it
 // (fapi::build_fw_queue), checked against the launch sequence: segments
-// tile [0, T) in launch order, and each one's `need` is the number of its
-// job's tasks in earlier launches.
+// tile [0, T); each job's segments are its phases in launch order; each
+// one's `need` is the number of its job's tasks in earlier phases.
 static bool queue_list(const fapi::FwBatchDesc &desc, int count, std::vector<PTask> *out) {
   std::vector<fapk::BatchQSeg> q;
   const int32_t T = fapi::build_fw_queue(desc, &q);
-  std::vector<int> before(count, 0);
-  size_t qi = 0;
-  int32_t first = 0;
-  for (size_t L = 0; L < desc.launch_tasks.size(); L++) {
-    for (int s = desc.seg_off[L]; s < desc.seg_off[L + 1]; s++, qi++) {
-      const fapk::BatchSeg &sg = desc.segs[s];
-      if (qi >= q.size()) {
-        printf("queue: %zu segments, launches hold more\n", q.size());
-        return false;
-      }
-      const fapk::BatchQSeg &g = q[qi];
-      if (g.job != sg.job || g.kind_r != sg.kind_r || g.count != sg.count ||
-          g.first != first || g.need != before[sg.job]) {
-        printf("queue segment %zu: job %d kind_r %d first %d count %d need %d; expected job %d "
-               "kind_r %d first %d count %d need %d\n", qi, g.job, g.kind_r, g.first, g.count,
-               g.need, sg.job, sg.kind_r, first, sg.count, before[sg.job]);
-        return false;
-      }
-      for (int idx = 0; idx < g.count; idx++)
-        out->push_back(PTask{g.job, g.kind_r & 7, g.kind_r >> 3, idx, g.need});
-      first += g.count;
-    }
+  std::vector<std::vector<fapk::BatchSeg>> phases(count);
+  for (size_t L = 0; L < desc.launch_tasks.size(); L++)
     for (int s = desc.seg_off[L]; s < desc.seg_off[L + 1]; s++)
-      before[desc.segs[s].job] += desc.segs[s].count;
+      phases[desc.segs[s].job].push_back(desc.segs[s]);
+  std::vector<size_t> next(count, 0);
+  std::vector<int> before(count, 0);
+  int32_t first = 0;
+  for (size_t qi = 0; qi < q.size(); qi++) {
+    const fapk::BatchQSeg &g = q[qi];
+    if (g.job < 0 || g.job >= count || next[g.job] >= phases[g.job].size()) {
+      printf("queue segment %zu: job %d out of range or past its phases\n", qi,
+             g.job);
+      return false;
+    }
+    const fapk::BatchSeg &want = phases[g.job][next[g.job]];
+    if (g.kind_r != want.kind_r || g.count != want.count || g.first != first ||
+        g.need != before[g.job]) {
+      printf("queue segment %zu: job %d kind_r %d first %d count %d need %d; "
+             "expected kind_r %d first %d count %d need %d\n", qi, g.job, g.kind_r,
+             g.first, g.count, g.need, want.kind_r, first, want.count, before[g.job]);
+      return false;
+    }
+    for (int idx = 0; idx < g.count; idx++)
+      out->push_back(PTask{g.job, g.kind_r & 7, g.kind_r >> 3, idx, g.need});
+    next[g.job]++;
+    before[g.job] += g.count;
+    first += g.count;
   }
-  if (qi != q.size() || first != T) {
-    printf("queue: %zu segments / %d tasks, launches give %zu / %d\n", q.size(), T, qi, first);
+  for (int j = 0; j < count; j++)
+    if (next[j] != phases[j].size()) {
+      printf("queue: job %d has %zu of its %zu phases\n", j, next[j],
+             phases[j].size());
+      return false;
+    }
+  if (first != T) {
+    printf("queue: %d tasks, builder says %d\n", first, T);
     return false;
   }
   return true;
