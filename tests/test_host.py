@@ -191,8 +191,10 @@ def test_batched_closure_schedule_simulated_on_cpu(tmp_path):
     one persistent launch with in-kernel waits on per-job phase counters
     under in-order dispatch into R resident slots: a static task deal with
     more workgroups than slots must be found stuck, a bounded wait must end
-    it with the error flag, and in-order task claims must drain for any
-    grid, with every matrix equal to plain Floyd-Warshall."""
+    it with the error flag, and in-order task claims (also claimed one task
+    ahead, as fw_queue_kernel does) over the library's own single-launch
+    queue (fapi::build_fw_queue, checked against the launch sequence) must
+    drain for any grid, with every matrix equal to plain Floyd-Warshall."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
