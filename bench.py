@@ -897,8 +897,9 @@ def batch_section(args, ctx, lib, ceiling):
     queues, paired pivot rounds, two chains), replayed from a hipGraph;
     persistent = every phase of every graph in ONE launch whose workgroups
     claim tasks in queue order and wait in the kernel for their graph's
-    earlier phases (fap_fw_batched_persistent, synchronous, descriptors
-    built per call).  Rows are 128-byte aligned (ld a multiple of 32), as
+    earlier phases (a persistent plan, fap_fw_plan_create_persistent:
+    queues built once, each run a replayed hipGraph of a counter reset and
+    the launch; fap_fw_plan_check after the timed runs).  Rows are 128-byte aligned (ld a multiple of 32), as
     the single launch requires.  Checksums of all matrices must agree."""
     import fastapsp_amd as fa
     from fastapsp_amd import synth
@@ -933,9 +934,10 @@ def batch_section(args, ctx, lib, ceiling):
         plan.run(ctx.stream)
 
     ns = [int(x) for x in sizes]
+    pplan = ctx.fw_plan(ptrs, ns, lds, use_graph=True, persistent=True)
 
     def persistent():
-        ctx.fw_batched_persistent(ptrs, ns, lds, ctx.stream)
+        pplan.run(ctx.stream)
 
     res = {}
     runs = (("batched", batched),) if args.batched_only else (
@@ -953,9 +955,12 @@ def batch_section(args, ctx, lib, ceiling):
             ctx.sync()
             t_ms.append(1e3 * (time.perf_counter() - t0))
         res[name] = float(np.median(t_ms))
+        if name == "persistent":
+            pplan.check()   # no wait gave up in any run
         res[name + "_sum"] = ctx.checksum(pool.ptr, total)
         log(f"[bench] c4 {name}: {res[name]:.2f} ms")
     plan.close()
+    pplan.close()
     for g in graphs:
         g.close()
     pool.free()

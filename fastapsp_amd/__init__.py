@@ -94,6 +94,9 @@ _SIGS = {
     "fap_fw_plan_create": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64), _int,
                                   C.POINTER(_vp)]),
     "fap_fw_plan_run": (_int, [_vp, _vp]),
+    "fap_fw_plan_create_persistent": (_int, [_vp, _i32, C.POINTER(_vp), _pi32, C.POINTER(_i64),
+                                             _int, C.POINTER(_vp)]),
+    "fap_fw_plan_check": (_int, [_vp]),
     "fap_fw_plan_info": (_int, [_vp, _pi32, C.POINTER(_i64), C.POINTER(_i64)]),
     "fap_fw_plan_destroy": (_int, [_vp]),
     "fap_dense_from_graphs": (_int, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i64),
@@ -452,15 +455,18 @@ class Context:
                                                L.ctypes.data_as(C.POINTER(_i64)), stream),
               "fap_fw_batched_persistent")
 
-    def fw_plan(self, ptrs, ns, lds, use_graph=True):
+    def fw_plan(self, ptrs, ns, lds, use_graph=True, persistent=False):
+        """A reusable batched closure: phase launches (fap_fw_plan_create) or,
+        with persistent, one persistent launch per run
+        (fap_fw_plan_create_persistent)."""
         k = len(ptrs)
         P = (_vp * k)(*ptrs)
         N = np.ascontiguousarray(ns, dtype=np.int32)
         L = np.ascontiguousarray(lds, dtype=np.int64)
         h = _vp()
-        check(load().fap_fw_plan_create(self.handle, k, P, _p32(N),
-                                        L.ctypes.data_as(C.POINTER(_i64)), int(use_graph),
-                                        C.byref(h)), "fap_fw_plan_create")
+        fn = "fap_fw_plan_create_persistent" if persistent else "fap_fw_plan_create"
+        check(getattr(load(), fn)(self.handle, k, P, _p32(N), L.ctypes.data_as(C.POINTER(_i64)),
+                                  int(use_graph), C.byref(h)), fn)
         return FwPlan(h.value)
 
     def dense_from_graphs(self, graphs, ptrs, lds, stream=None):
@@ -483,6 +489,10 @@ class FwPlan:
 
     def run(self, stream=None):
         check(load().fap_fw_plan_run(self.handle, stream), "fap_fw_plan_run")
+
+    def check(self):
+        """Waits for the last run; raises if a persistent run gave up."""
+        check(load().fap_fw_plan_check(self.handle), "fap_fw_plan_check")
 
     def info(self):
         a, b, c = C.c_int32(), C.c_int64(), C.c_int64()

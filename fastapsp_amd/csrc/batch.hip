@@ -364,6 +364,209 @@ int32_t build_fw_queue(const FwBatchDesc &desc, std::vector<fapk::BatchQSeg> *ou
   return first;
 }
 
+// XCDs of the context's device (bit mask), probed once: a grid of a few
+// workgroups per CU records the XCC ids it runs on.
+static int device_xcc_mask(fap_ctx *ctx, hipStream_t s, int *mask) {
+  if (ctx->xcc_mask <= 0) {
+    int32_t *d = nullptr, h = 0;
+    FAP_HIP(hipMalloc(&d, 256));
+    hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3(1), dim3(64), 0, s, d, 1,
+                       (int32_t *)nullptr, 0);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(fapk::xcc_probe_kernel, dim3(std::max(ctx->num_cus, 1) * 4), dim3(64), 0,
+                         s, d);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "XCD probe", __FILE__, __LINE__);
+    if (h <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: XCD probe saw no XCC id");
+    ctx->xcc_mask = h;
+  }
+  *mask = ctx->xcc_mask;
+  return FAP_OK;
+}
+
+// Host side of a persistent launch (fw_queue_kernel): jobs split over the
+// device's XCDs by work (2 n^3, largest first, each to the least loaded),
+// one queue per XCD; one device block holds
+//   jobs | segments | qinfo | segment map offsets | local segment ids
+//   (uploaded) | ctrl | done | task -> segment map.
+int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                      const int64_t *ld, hipStream_t s, QueueBlock *qb) {
+  // tasks of one phase write distinct tiles, and a tile's rows must not
+  // share a 128-byte cache line with another tile's
+  for (int32_t j = 0; j < count; j++)
+    if (n[j] > 0 && ((((uintptr_t)d_mats[j]) & 127) != 0 || (ld[j] & 31) != 0))
+      return fail(FAP_E_ARG, "fw_batched_persistent: matrix " + std::to_string(j) +
+                                 " needs 128-byte aligned rows (base address a multiple of 128, "
+                                 "ld a multiple of 32)");
+  FAP_HIP(hipSetDevice(ctx->device));
+  int khz = 0;
+  FAP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: no wall clock rate");
+  qb->wait_us = ctx->opt_batch_wait_us > 0 ? ctx->opt_batch_wait_us : 2000000;
+  qb->limit = qb->wait_us * khz / 1000;
+  int mask = 0;
+  FAP_TRY(device_xcc_mask(ctx, s, &mask));
+  std::vector<int> xccs;
+  for (int x = 0; x < 16; x++)
+    if (mask & (1 << x)) xccs.push_back(x);
+  const int nq = (int)std::min<size_t>(xccs.size(), (size_t)count);
+
+  std::vector<int32_t> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  std::vector<double> load(nq, 0.0);
+  std::vector<std::vector<int32_t>> qjobs(nq);
+  for (int j : order) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    qjobs[q].push_back(j);
+    load[q] += 2.0 * n[j] * (double)n[j] * n[j] + 1.0;
+  }
+  // per queue: its own launch sequence laid end to end; jobs and segments
+  // concatenated over the queues (segment job indices made global)
+  std::vector<fapk::BatchJob> jobs;
+  std::vector<fapk::BatchQSeg> segs;
+  std::vector<int32_t> qinfo(16 + 4 * nq, 0), seg_map_off, seg_local;
+  qb->job_total.clear();
+  qb->ops = 0;
+  for (int x = 0; x < 16; x++) qinfo[x] = -1;
+  int64_t ntasks = 0;
+  for (int q = 0; q < nq; q++) {
+    qinfo[xccs[q]] = q;
+    std::vector<int32_t *> m;
+    std::vector<int32_t> nn;
+    std::vector<int64_t> ll;
+    for (int j : qjobs[q]) {
+      m.push_back(d_mats[j]);
+      nn.push_back(n[j]);
+      ll.push_back(ld[j]);
+    }
+    FwBatchDesc d;
+    build_fw_batch((int32_t)m.size(), m.data(), nn.data(), ll.data(), &d);
+    qb->ops += d.ops;
+    std::vector<fapk::BatchQSeg> qs;
+    const int32_t qt = build_fw_queue(d, &qs);
+    if (ntasks + qt + (int64_t)ctx->num_cus * 8 >= INT32_MAX)
+      return fail(FAP_E_ARG, "fw_batched_persistent: too many tasks");
+    const int32_t job_base = (int32_t)jobs.size();
+    qinfo[16 + 4 * q] = (int32_t)segs.size();
+    qinfo[16 + 4 * q + 1] = (int32_t)qs.size();
+    qinfo[16 + 4 * q + 2] = qt;
+    qinfo[16 + 4 * q + 3] = (int32_t)ntasks;
+    for (const fapk::BatchJob &jb : d.jobs) {
+      jobs.push_back(jb);
+      qb->job_total.push_back(0);
+    }
+    for (fapk::BatchQSeg g : qs) {
+      g.job += job_base;
+      qb->job_total[g.job] += g.count;
+      seg_map_off.push_back((int32_t)ntasks);
+      seg_local.push_back((int32_t)(segs.size() - qinfo[16 + 4 * q]));
+      segs.push_back(g);
+    }
+    ntasks += qt;
+  }
+  qb->ntasks = ntasks;
+  qb->njobs = (int)jobs.size();
+  qb->nseg = (int)segs.size();
+  qb->nctrl = 32 * (nq + 1);
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t jobs_b = sizeof(fapk::BatchJob) * jobs.size();
+  qb->seg_off = up(jobs_b);
+  qb->qi_off = up(qb->seg_off + sizeof(fapk::BatchQSeg) * segs.size());
+  qb->smo_off = up(qb->qi_off + sizeof(int32_t) * qinfo.size());
+  qb->sl_off = up(qb->smo_off + sizeof(int32_t) * segs.size());
+  qb->ctrl_off = up(qb->sl_off + sizeof(int32_t) * segs.size());
+  qb->done_off = qb->ctrl_off + sizeof(int32_t) * qb->nctrl;
+  qb->map_off = up(qb->done_off + sizeof(int32_t) * std::max(qb->njobs, 1));
+  qb->total = qb->map_off + sizeof(int32_t) * (size_t)std::max<int64_t>(ntasks, 1);
+  qb->host.assign(qb->ctrl_off, 0);
+  char *h = qb->host.data();
+  if (jobs_b) memcpy(h, jobs.data(), jobs_b);
+  if (!segs.empty()) {
+    memcpy(h + qb->seg_off, segs.data(), sizeof(fapk::BatchQSeg) * segs.size());
+    memcpy(h + qb->smo_off, seg_map_off.data(), sizeof(int32_t) * segs.size());
+    memcpy(h + qb->sl_off, seg_local.data(), sizeof(int32_t) * segs.size());
+  }
+  memcpy(h + qb->qi_off, qinfo.data(), sizeof(int32_t) * qinfo.size());
+  qb->grid = ctx->num_cus * 6;  // 6 resident workgroups per CU; dealt to the XCDs in turn
+  return FAP_OK;
+}
+
+// Descriptors up and the task -> segment map filled (once per block).
+int upload_queue_block(const QueueBlock &qb, char *dev, hipStream_t s) {
+  FAP_HIP(hipMemcpyAsync(dev, qb.host.data(), qb.ctrl_off, hipMemcpyHostToDevice, s));
+  if (qb.nseg > 0) {
+    hipLaunchKernelGGL(fapk::fw_queue_map_kernel, dim3(std::min(qb.nseg, 65535)), dim3(256), 0, s,
+                       (const fapk::BatchQSeg *)(dev + qb.seg_off),
+                       (const int32_t *)(dev + qb.smo_off), (const int32_t *)(dev + qb.sl_off),
+                       qb.nseg, (int32_t *)(dev + qb.map_off));
+    FAP_CHECK_LAUNCH();
+  }
+  return FAP_OK;
+}
+
+// One closure of the batch: counters cleared, then the persistent launch
+// (kernels only, so the pair can be captured in a graph).
+int enqueue_queue_run(const QueueBlock &qb, char *dev, hipStream_t s) {
+  if (qb.ntasks <= 0) return FAP_OK;
+  int32_t *ctrl = (int32_t *)(dev + qb.ctrl_off), *done = (int32_t *)(dev + qb.done_off);
+  hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3((std::max(qb.nctrl, qb.njobs) + 255) / 256),
+                     dim3(256), 0, s, ctrl, qb.nctrl, done, qb.njobs);
+  FAP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(fapk::fw_queue_kernel, dim3(qb.grid), dim3(fapk::MP_THREADS), 0, s,
+                     (const fapk::BatchJob *)dev, (const fapk::BatchQSeg *)(dev + qb.seg_off),
+                     (const int32_t *)(dev + qb.qi_off), (const int32_t *)(dev + qb.map_off), ctrl,
+                     done, qb.limit);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+// After a closure has finished: the give-up flag and every job's count.
+int check_queue_run(const QueueBlock &qb, const char *dev) {
+  if (qb.ntasks <= 0) return FAP_OK;
+  std::vector<int32_t> got(qb.njobs + 1, 0);
+  FAP_HIP(hipMemcpy(got.data(), dev + qb.done_off, sizeof(int32_t) * qb.njobs,
+                    hipMemcpyDeviceToHost));
+  FAP_HIP(hipMemcpy(got.data() + qb.njobs, dev + qb.ctrl_off, sizeof(int32_t),
+                    hipMemcpyDeviceToHost));
+  if (got[qb.njobs] != 0)
+    return fail(FAP_E_DEVICE, "fw_batched_persistent: a task waited longer than " +
+                                  std::to_string((long long)qb.wait_us) +
+                                  " us for its job's earlier phases; the launch gave up "
+                                  "(matrices partly closed)");
+  for (int j = 0; j < qb.njobs; j++)
+    if (got[j] != qb.job_total[j])
+      return fail(FAP_E_DEVICE, "fw_batched_persistent: a matrix's queue ran " +
+                                    std::to_string(got[j]) + " of its " +
+                                    std::to_string(qb.job_total[j]) +
+                                    " tasks (no workgroup on its XCD)");
+  return FAP_OK;
+}
+
+// The batch in one launch, synchronously.
+static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                        const int64_t *ld, hipStream_t s) {
+  QueueBlock qb;
+  FAP_TRY(build_queue_block(ctx, count, d_mats, n, ld, s, &qb));
+  if (qb.ntasks <= 0) return FAP_OK;
+  char *dev = nullptr;
+  FAP_HIP(hipMalloc(&dev, qb.total));
+  int rc = upload_queue_block(qb, dev, s);
+  if (rc == FAP_OK) rc = enqueue_queue_run(qb, dev, s);
+  if (rc == FAP_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "fw_batched_persistent", __FILE__, __LINE__);
+  }
+  if (rc == FAP_OK) rc = check_queue_run(qb, dev);
+  (void)hipFree(dev);
+  return rc;
+}
+
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,
                     const fapk::BatchSeg *d_segs, hipStream_t s, size_t first, size_t last) {
   const int cap = ctx->num_cus * batch_grid_per_cu(ctx, desc.key);
@@ -422,6 +625,9 @@ struct fap_fw_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t ops = 0;
+  fapi::QueueBlock *queue = nullptr;  // persistent plan: one launch per run
+  char *qdev = nullptr;               // its device block
+  int64_t runs = 0;
 };
 
 namespace fapi {
@@ -434,6 +640,7 @@ static int chain_launch(fap_fw_plan *p, int c, hipStream_t s, size_t first = 0,
 }
 
 static int plan_launch(fap_fw_plan *p, hipStream_t s) {
+  if (p->queue) return enqueue_queue_run(*p->queue, p->qdev, s);
   if (p->chains == 1) return chain_launch(p, 0, s);
   // fork: the side stream starts after what precedes the plan on s, and
   // after the first chain's first PLAN_CHAIN_OFFSET launches; join: s waits
@@ -471,6 +678,8 @@ void fw_plan_destroy(fap_fw_plan *p) {
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_offset) (void)hipEventDestroy(p->ev_offset);
   if (p->ev_last) (void)hipEventDestroy(p->ev_last);
+  if (p->qdev) (void)hipFree(p->qdev);
+  delete p->queue;
   delete p;
 }
 
@@ -553,6 +762,46 @@ int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int3
   return FAP_OK;
 }
 
+// A plan of one persistent launch per run: the queue block built and
+// uploaded (task -> segment map filled) once; with use_graph the reset and
+// the launch are captured.
+int fw_plan_create_persistent(fap_ctx *ctx, int32_t count, int32_t *const *mats,
+                              const int32_t *n, const int64_t *ld, int use_graph,
+                              fap_fw_plan **out) {
+  fap_fw_plan *p = new fap_fw_plan();
+  p->ctx = ctx;
+  p->queue = new QueueBlock();
+  int rc = FAP_OK;
+  hipError_t e = hipEventCreateWithFlags(&p->ev_last, hipEventDisableTiming);
+  if (e != hipSuccess) rc = hip_fail(e, "batch plan event", __FILE__, __LINE__);
+  if (rc == FAP_OK) rc = build_queue_block(ctx, count, mats, n, ld, ctx->stream, p->queue);
+  if (rc == FAP_OK) {
+    p->ops = p->queue->ops;
+    e = hipMalloc(&p->qdev, p->queue->total);
+    if (e != hipSuccess) rc = hip_fail(e, "persistent plan block", __FILE__, __LINE__);
+  }
+  if (rc == FAP_OK) rc = upload_queue_block(*p->queue, p->qdev, ctx->stream);
+  if (rc == FAP_OK) {
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "persistent plan upload", __FILE__, __LINE__);
+  }
+  if (rc == FAP_OK && use_graph && p->queue->ntasks > 0) {
+    hipStream_t cs = ctx->aux_stream;  // capture on a stream with no other work queued
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) rc = plan_launch(p, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess && rc == FAP_OK) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (rc == FAP_OK && e != hipSuccess) rc = hip_fail(e, "persistent plan graph capture", __FILE__, __LINE__);
+  }
+  if (rc != FAP_OK) {
+    fw_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return FAP_OK;
+}
+
 int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
   if (p->exec) {
     FAP_HIP(hipGraphLaunch(p->exec, s));
@@ -560,6 +809,7 @@ int fw_plan_run(fap_fw_plan *p, hipStream_t s) {
     FAP_TRY(plan_launch(p, s));
   }
   FAP_HIP(hipEventRecord(p->ev_last, s));
+  p->runs++;
   return FAP_OK;
 }
 
@@ -611,173 +861,6 @@ static int check_jobs(int32_t count, int32_t *const *d_mats, const int32_t *n, c
 }  // namespace fapi
 
 using namespace fapi;
-
-// XCDs of the context's device (bit mask), probed once: a grid of a few
-// workgroups per CU records the XCC ids it runs on.
-static int device_xcc_mask(fap_ctx *ctx, hipStream_t s, int *mask) {
-  if (ctx->xcc_mask <= 0) {
-    int32_t *d = nullptr, h = 0;
-    FAP_HIP(hipMalloc(&d, 256));
-    hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3(1), dim3(64), 0, s, d, 1,
-                       (int32_t *)nullptr, 0);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(fapk::xcc_probe_kernel, dim3(std::max(ctx->num_cus, 1) * 4), dim3(64), 0,
-                         s, d);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(d);
-    if (e != hipSuccess) return hip_fail(e, "XCD probe", __FILE__, __LINE__);
-    if (h <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: XCD probe saw no XCC id");
-    ctx->xcc_mask = h;
-  }
-  *mask = ctx->xcc_mask;
-  return FAP_OK;
-}
-
-// The batch in one launch (fw_queue_kernel): jobs split over the device's
-// XCDs by work (2 n^3, largest first, each to the least loaded), one queue
-// per XCD; then the flag and every job's count.
-static int run_fw_queue(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
-                        const int64_t *ld, hipStream_t s) {
-  // tasks of one phase write distinct tiles, and a tile's rows must not
-  // share a 128-byte cache line with another tile's
-  for (int32_t j = 0; j < count; j++)
-    if (n[j] > 0 && ((((uintptr_t)d_mats[j]) & 127) != 0 || (ld[j] & 31) != 0))
-      return fail(FAP_E_ARG, "fw_batched_persistent: matrix " + std::to_string(j) +
-                                 " needs 128-byte aligned rows (base address a multiple of 128, "
-                                 "ld a multiple of 32)");
-  FAP_HIP(hipSetDevice(ctx->device));
-  int khz = 0;
-  FAP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
-  if (khz <= 0) return fail(FAP_E_HIP, "fw_batched_persistent: no wall clock rate");
-  const int64_t wait_us = ctx->opt_batch_wait_us > 0 ? ctx->opt_batch_wait_us : 2000000;
-  const int64_t limit = wait_us * khz / 1000;
-  int mask = 0;
-  FAP_TRY(device_xcc_mask(ctx, s, &mask));
-  std::vector<int> xccs;
-  for (int x = 0; x < 16; x++)
-    if (mask & (1 << x)) xccs.push_back(x);
-  const int nq = (int)std::min<size_t>(xccs.size(), (size_t)count);
-
-  // jobs to queues
-  std::vector<int32_t> order(count);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
-  std::vector<double> load(nq, 0.0);
-  std::vector<std::vector<int32_t>> qjobs(nq);
-  for (int j : order) {
-    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    qjobs[q].push_back(j);
-    load[q] += 2.0 * n[j] * (double)n[j] * n[j] + 1.0;
-  }
-  // per queue: its own launch sequence laid end to end; jobs and segments
-  // concatenated over the queues (segment job indices made global)
-  std::vector<fapk::BatchJob> jobs;
-  std::vector<fapk::BatchQSeg> segs;
-  std::vector<int32_t> qinfo(16 + 4 * nq, 0), job_total, seg_map_off, seg_local;
-  for (int x = 0; x < 16; x++) qinfo[x] = -1;
-  int64_t ntasks = 0;
-  for (int q = 0; q < nq; q++) {
-    qinfo[xccs[q]] = q;
-    std::vector<int32_t *> m;
-    std::vector<int32_t> nn;
-    std::vector<int64_t> ll;
-    for (int j : qjobs[q]) {
-      m.push_back(d_mats[j]);
-      nn.push_back(n[j]);
-      ll.push_back(ld[j]);
-    }
-    FwBatchDesc d;
-    build_fw_batch((int32_t)m.size(), m.data(), nn.data(), ll.data(), &d);
-    std::vector<fapk::BatchQSeg> qs;
-    const int32_t qt = build_fw_queue(d, &qs);
-    if (ntasks + qt + (int64_t)ctx->num_cus * 8 >= INT32_MAX)
-      return fail(FAP_E_ARG, "fw_batched_persistent: too many tasks");
-    const int32_t job_base = (int32_t)jobs.size();
-    qinfo[16 + 4 * q] = (int32_t)segs.size();
-    qinfo[16 + 4 * q + 1] = (int32_t)qs.size();
-    qinfo[16 + 4 * q + 2] = qt;
-    qinfo[16 + 4 * q + 3] = (int32_t)ntasks;
-    for (const fapk::BatchJob &jb : d.jobs) {
-      jobs.push_back(jb);
-      job_total.push_back(0);
-    }
-    for (fapk::BatchQSeg g : qs) {
-      g.job += job_base;
-      job_total[g.job] += g.count;
-      seg_map_off.push_back((int32_t)ntasks);
-      seg_local.push_back((int32_t)(segs.size() - qinfo[16 + 4 * q]));
-      segs.push_back(g);
-    }
-    ntasks += qt;
-  }
-  if (ntasks == 0) return FAP_OK;
-  const int njobs = (int)jobs.size();
-  const size_t jobs_b = sizeof(fapk::BatchJob) * jobs.size();
-  const size_t seg_off = (jobs_b + 255) & ~(size_t)255;
-  const size_t qi_off = (seg_off + sizeof(fapk::BatchQSeg) * segs.size() + 255) & ~(size_t)255;
-  const size_t smo_off = (qi_off + sizeof(int32_t) * qinfo.size() + 255) & ~(size_t)255;
-  const size_t sl_off = (smo_off + sizeof(int32_t) * segs.size() + 255) & ~(size_t)255;
-  const size_t ctrl_off = (sl_off + sizeof(int32_t) * segs.size() + 255) & ~(size_t)255;
-  const int nctrl = 32 * (nq + 1);
-  const size_t done_off = ctrl_off + sizeof(int32_t) * nctrl;
-  const size_t map_off = (done_off + sizeof(int32_t) * njobs + 255) & ~(size_t)255;
-  const size_t total = map_off + sizeof(int32_t) * (size_t)ntasks;
-  std::vector<char> host(ctrl_off, 0);
-  memcpy(host.data(), jobs.data(), jobs_b);
-  memcpy(host.data() + seg_off, segs.data(), sizeof(fapk::BatchQSeg) * segs.size());
-  memcpy(host.data() + qi_off, qinfo.data(), sizeof(int32_t) * qinfo.size());
-  memcpy(host.data() + smo_off, seg_map_off.data(), sizeof(int32_t) * segs.size());
-  memcpy(host.data() + sl_off, seg_local.data(), sizeof(int32_t) * segs.size());
-  char *dev = nullptr;
-  FAP_HIP(hipMalloc(&dev, total));
-  int32_t *ctrl = (int32_t *)(dev + ctrl_off), *done = (int32_t *)(dev + done_off);
-  std::vector<int32_t> got(njobs + 1, 0);
-  hipError_t e = hipMemcpyAsync(dev, host.data(), ctrl_off, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(fapk::fw_queue_reset_kernel, dim3((std::max(nctrl, njobs) + 255) / 256),
-                       dim3(256), 0, s, ctrl, nctrl, done, njobs);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(fapk::fw_queue_map_kernel, dim3(std::min<size_t>(segs.size(), 65535)),
-                       dim3(256), 0, s, (const fapk::BatchQSeg *)(dev + seg_off),
-                       (const int32_t *)(dev + smo_off), (const int32_t *)(dev + sl_off),
-                       (int)segs.size(), (int32_t *)(dev + map_off));
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    // 6 resident workgroups per CU; workgroups are dealt to the XCDs in turn
-    const int grid = ctx->num_cus * 6;
-    hipLaunchKernelGGL(fapk::fw_queue_kernel, dim3(grid), dim3(fapk::MP_THREADS), 0, s,
-                       (const fapk::BatchJob *)dev, (const fapk::BatchQSeg *)(dev + seg_off),
-                       (const int32_t *)(dev + qi_off), (const int32_t *)(dev + map_off), ctrl,
-                       done, limit);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(got.data(), done, sizeof(int32_t) * njobs,
-                                          hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(got.data() + njobs, ctrl, sizeof(int32_t), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(dev);
-  if (e != hipSuccess) return hip_fail(e, "fw_batched_persistent", __FILE__, __LINE__);
-  if (got[njobs] != 0)
-    return fail(FAP_E_DEVICE, "fw_batched_persistent: a task waited longer than " +
-                                  std::to_string((long long)wait_us) +
-                                  " us for its job's earlier phases; the launch gave up "
-                                  "(matrices partly closed)");
-  for (int j = 0; j < njobs; j++)
-    if (got[j] != job_total[j])
-      return fail(FAP_E_DEVICE, "fw_batched_persistent: a matrix's queue ran " +
-                                    std::to_string(got[j]) + " of its " +
-                                    std::to_string(job_total[j]) +
-                                    " tasks (no workgroup on its XCD)");
-  return FAP_OK;
-}
 
 extern "C" {
 
@@ -843,6 +926,22 @@ int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, cons
   return fw_plan_create(ctx, count, d_mats, n, ld, use_graph, out);
 }
 
+int fap_fw_plan_create_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mats,
+                                  const int32_t *n, const int64_t *ld, int use_graph,
+                                  fap_fw_plan **out) {
+  if (!ctx || !out) return fail(FAP_E_ARG, "null argument");
+  FAP_TRY(check_jobs(count, d_mats, n, ld));
+  return fw_plan_create_persistent(ctx, count, d_mats, n, ld, use_graph, out);
+}
+
+int fap_fw_plan_check(fap_fw_plan *plan) {
+  if (!plan) return fail(FAP_E_ARG, "null plan");
+  if (plan->runs == 0) return FAP_OK;
+  FAP_HIP(hipEventSynchronize(plan->ev_last));
+  if (plan->queue) return check_queue_run(*plan->queue, plan->qdev);
+  return FAP_OK;
+}
+
 int fap_fw_plan_run(fap_fw_plan *plan, void *stream) {
   if (!plan) return fail(FAP_E_ARG, "null plan");
   return fw_plan_run(plan, pick_stream(plan->ctx, stream));
@@ -852,7 +951,11 @@ int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks,
   if (!plan) return fail(FAP_E_ARG, "null plan");
   int64_t t = 0;
   int32_t l = 0;
-  for (int c = 0; c < plan->chains; c++) {
+  if (plan->queue) {
+    t = plan->queue->ntasks;
+    l = t > 0 ? 1 : 0;
+  }
+  for (int c = 0; c < plan->chains && !plan->queue; c++) {
     for (int32_t x : plan->host[c].launch_tasks) t += x;
     l += (int32_t)plan->host[c].launch_tasks.size();
   }

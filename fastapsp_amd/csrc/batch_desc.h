@@ -159,6 +159,23 @@ void build_fw_batch(int32_t count, int32_t *const *mats, const int32_t *n, const
 // The launches of `desc` as one queue, launch after launch (so a task's
 // own job's earlier phases always come before it); returns the task count.
 int32_t build_fw_queue(const FwBatchDesc &desc, std::vector<fapk::BatchQSeg> *out);
+// Host side of a batch's persistent launch (fw_queue_kernel): the
+// descriptor bytes to upload (host, the first ctrl_off bytes of the device
+// block) and the block's layout.
+struct QueueBlock {
+  std::vector<char> host;
+  size_t seg_off = 0, qi_off = 0, smo_off = 0, sl_off = 0, ctrl_off = 0, done_off = 0,
+         map_off = 0, total = 0;
+  int nctrl = 0, njobs = 0, nseg = 0, grid = 0;
+  int64_t ntasks = 0, ops = 0, limit = 0, wait_us = 0;
+  std::vector<int32_t> job_total;  // tasks per job (block order)
+};
+int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
+                      const int64_t *ld, hipStream_t s, QueueBlock *qb);
+int upload_queue_block(const QueueBlock &qb, char *dev, hipStream_t s);
+int enqueue_queue_run(const QueueBlock &qb, char *dev, hipStream_t s);
+int check_queue_run(const QueueBlock &qb, const char *dev);
+
 // Launch sequence of a batch whose jobs and segments are already on the
 // device (same order as in `desc`).
 int launch_fw_batch(fap_ctx *ctx, const FwBatchDesc &desc, const fapk::BatchJob *d_jobs,

@@ -199,6 +199,18 @@ int fap_fw_batched_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mat
 typedef struct fap_fw_plan fap_fw_plan;
 int fap_fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                        const int64_t *ld, int use_graph, fap_fw_plan **out);
+/* A plan whose every run is ONE persistent launch (as
+ * fap_fw_batched_persistent, same requirements): queues, descriptors and
+ * the task map uploaded once; with use_graph each run replays a captured
+ * hipGraph of a counter reset and the launch.  Runs are asynchronous: a
+ * wait that gave up (or a queue left unfinished) is reported by
+ * fap_fw_plan_check, which waits for the plan's last run. */
+int fap_fw_plan_create_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mats,
+                                  const int32_t *n, const int64_t *ld, int use_graph,
+                                  fap_fw_plan **out);
+/* Waits for the plan's last run; FAP_E_DEVICE if it was a persistent run
+ * that gave up or left a matrix unfinished, FAP_OK otherwise. */
+int fap_fw_plan_check(fap_fw_plan *plan);
 int fap_fw_plan_run(fap_fw_plan *plan, void *stream);
 int fap_fw_plan_info(const fap_fw_plan *plan, int32_t *launches, int64_t *tasks, int64_t *ops);
 int fap_fw_plan_destroy(fap_fw_plan *plan);

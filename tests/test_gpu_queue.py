@@ -89,6 +89,17 @@ def test_persistent_matches_phase_launches_on_512_rggs(gpu_ctx):
         ctx.sync()
         ctx.fw_batched_persistent(ptrs, ns, lds)
         assert ctx.checksum(pool.ptr, total) == want, f"single launch, run {rep}"
+    plan = ctx.fw_plan(ptrs, ns, lds, use_graph=True, persistent=True)
+    try:
+        assert plan.info()["launches"] == 1
+        for rep in range(2):   # graph replays, eager work in between
+            ctx.dense_from_graphs(graphs, ptrs, lds)
+            ctx.sync()
+            plan.run(ctx.stream)
+            plan.check()
+            assert ctx.checksum(pool.ptr, total) == want, f"persistent plan, replay {rep}"
+    finally:
+        plan.close()
     rng = np.random.default_rng(45)
     for j in rng.choice(len(ns), 12, replace=False):
         n = ns[j]
@@ -136,4 +147,55 @@ def test_persistent_wait_limit_gives_up_and_context_recovers(gpu_ctx):
     for n, m, d in zip(sizes, mats, devs):
         got = d.get().reshape(n, _ld32(n))
         np.testing.assert_array_equal(got[:, :n], O.floyd(m), err_msg=f"n={n}")
+        d.free()
+
+
+@pytest.mark.parametrize("sizes", [[0, 70, 0, 1], [640], [5, 17, 33, 64, 2, 9, 40, 63, 11, 64, 1]])
+def test_persistent_edge_batches(gpu_ctx, sizes):
+    """Empty matrices (null pointers allowed), one matrix (one queue: the
+    workgroups of the other XCDs exit at once), and more matrices than XCDs
+    that are all single-block (every task a diagonal with nothing to wait
+    for)."""
+    print(f"[queue] sizes {sizes}", flush=True)
+    live = [n for n in sizes if n > 0]
+    mats, devs = _matrices(gpu_ctx, live, 900)
+    it = iter(devs)
+    ptrs = [next(it).ptr if n > 0 else 0 for n in sizes]
+    gpu_ctx.fw_batched_persistent(ptrs, sizes, [_ld32(n) if n > 0 else 0 for n in sizes])
+    for n, m, d in zip(live, mats, devs):
+        got = d.get().reshape(n, _ld32(n))
+        np.testing.assert_array_equal(got[:, :n], O.floyd(m), err_msg=f"n={n}")
+        assert (got[:, n:] == -7).all()
+        d.free()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_persistent_plan_replays_match_oracle(gpu_ctx, use_graph):
+    """A persistent plan (one launch per run, queues and task map uploaded
+    once), replayed three times on reinitialised matrices, with and without
+    its captured graph; check() after each run."""
+    sizes = [300, 64, 1, 129, 700, 65, 200, 33, 513]
+    print(f"[queue] plan sizes {sizes} graph={use_graph}", flush=True)
+    mats, devs = _matrices(gpu_ctx, sizes, 950)
+    ptrs, lds = [d.ptr for d in devs], [_ld32(n) for n in sizes]
+    want = [O.floyd(m) for m in mats]
+    plan = gpu_ctx.fw_plan(ptrs, sizes, lds, use_graph=use_graph, persistent=True)
+    try:
+        plan.check()   # no run yet: nothing to report
+        info = plan.info()
+        assert info["launches"] == 1 and info["ops"] == sum(2 * n ** 3 for n in sizes)
+        for rep in range(3):
+            for m, d, n in zip(mats, devs, sizes):
+                buf = np.full((n, _ld32(n)), -7, np.int32)
+                buf[:, :n] = m
+                d.set(buf)
+            plan.run(gpu_ctx.stream)
+            plan.check()
+            for n, w, d in zip(sizes, want, devs):
+                got = d.get().reshape(n, _ld32(n))
+                np.testing.assert_array_equal(got[:, :n], w, err_msg=f"replay {rep} n={n}")
+                assert (got[:, n:] == -7).all()
+    finally:
+        plan.close()
+    for d in devs:
         d.free()
