@@ -64,10 +64,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # spec: 1 int32 op per lane per clock, 39.32 T
-# HBM bytes per launch of the n = 8192 min-plus product, from two rocprofv3
-# counter runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a copy of
-# known size): 6.66e9 read + 0.27e9 written.
-MINPLUS_8192_TRAFFIC = 6.93e9
+# HBM bytes per launch of the n = 8192 min-plus product and per n = 8192
+# tile-128 FW closure, from two rocprofv3 counter runs (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, scales from a copy of known size): product 6.47e9 read
+# + 0.27e9 written; closure 34.07e9 read + 17.44e9 written.
+MINPLUS_8192_TRAFFIC = 6.74e9
+FW_8192_T128_TRAFFIC = 51.51e9
 HBM_PEAK_GBS = 8000.0
 
 
@@ -215,7 +217,9 @@ def dense_section(args, ctx, lib):
                      "kernel": "Floyd-Warshall closure (fw_rest phase-3 tiles with the fused "
                                "next diagonal, fw_panel, fw_diag; one closure per step)",
                      "achieved": round(ach_fw, 2), "peak": round(ceiling, 2), "unit": "TOP/s",
-                     "frac": round(ach_fw / ceiling, 3), "traffic": None,
+                     "frac": round(ach_fw / ceiling, 3),
+                     "traffic": (FW_8192_T128_TRAFFIC if n == 8192 and tuned
+                                 and tuned.get("params", [0])[0] == 128 else None),
                      "peak_kind": "measured v_add_u32 + v_min3_i32 mix ceiling",
                      "spec_peak": round(VALU_PEAK_TOPS, 2),
                      "frac_of_spec_peak": round(ach_fw / VALU_PEAK_TOPS, 3),
