@@ -190,35 +190,35 @@ __global__ void xcc_probe_kernel(int32_t *mask) {
 // tasks close their block with 256 threads (close_block64).  All task
 // kinds in one kernel take 80 VGPRs (64 spill): 6 waves per SIMD.
 //   qinfo: [0, 16) qmap (XCC id -> queue or -1); then per queue q, at
-//          16 + 4q: first segment, segment count, task count, first entry
-//          of its task -> segment map
-//   segmap: task -> segment (within its queue), filled by fw_queue_map_kernel
+//          16 + 4q: first segment, segment count, task count, first record
+//   taskrec: per task {job, kind_r, index in its phase, need}, filled by
+//          fw_queue_map_kernel (one load per claimed task)
 //   ctrl:  [0] flag; claim counter of queue q at 32 (q + 1) (own cache line)
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fw_queue_kernel(
-    const BatchJob *__restrict__ jobs, const BatchQSeg *__restrict__ all_segs,
-    const int32_t *__restrict__ qinfo, const int32_t *__restrict__ all_segmap, int32_t *ctrl,
-    int32_t *done, int64_t limit) {
+    const BatchJob *__restrict__ jobs, const int32_t *__restrict__ qinfo,
+    const int4 *__restrict__ all_rec, int32_t *ctrl, int32_t *done, int64_t limit) {
   constexpr int W = BLK_LDS_WORDS > MpSmall::LDS_WORDS ? BLK_LDS_WORDS : MpSmall::LDS_WORDS;
   __shared__ __attribute__((aligned(16))) int32_t lds[W];
   __shared__ int32_t s_next, s_go;
   const int t = threadIdx.x;
   const int q = qinfo[xcc_id()];
   if (q < 0) return;
-  const BatchQSeg *segs = all_segs + qinfo[16 + 4 * q];
   const int ntasks = qinfo[16 + 4 * q + 2];
-  const int32_t *segmap = all_segmap + qinfo[16 + 4 * q + 3];
+  const int4 *rec = all_rec + qinfo[16 + 4 * q + 3];
   int32_t *claim = ctrl + 32 * (q + 1);
   if (t == 0) s_next = atomicAdd(claim, 1);
   __syncthreads();
   int ti = s_next;
+  int nxt = ntasks;  // thread 0: the next claim, made while the current task runs
   while (ti < ntasks) {
-    const BatchQSeg sg = segs[segmap[ti]];
+    const int4 tr = rec[ti];  // {job, kind_r, index, need}
     if (t == 0) {
       int go = 1;
-      if (sg.need > 0) {
+      // the flag is read only while waiting: a task run after another
+      // workgroup gave up is harmless, the call reports the flag anyway
+      if (tr.w > 0) {
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        while (__hip_atomic_load(&done[sg.job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               sg.need) {
+        while (__hip_atomic_load(&done[tr.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tr.w) {
           if (__hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
               (int64_t)(__builtin_amdgcn_s_memrealtime() - t0) > limit) {
             go = 0;
@@ -227,48 +227,46 @@ __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
           __builtin_amdgcn_s_sleep(2);
         }
       }
-      if (go && __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-        go = 0;
       if (!go) {
         __hip_atomic_store(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_next = ntasks;
       } else {
-        s_next = atomicAdd(claim, 1);  // the next claim, while this task runs
+        // published after the task, so the claim's round trip overlaps it
+        nxt = atomicAdd(claim, 1);
         // the CU's L1 may hold lines of this job's tiles from before their
         // last writes: drop them (an agent-scope acquire; the barrier below
         // orders every wave's loads after it)
-        if (sg.need > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (tr.w > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       s_go = go;
     }
     __syncthreads();
-    const int next = s_next, go = s_go;  // read before thread 0 can write them again
-    if (!go) break;
-    const BatchJob job = jobs[sg.job];
-    const int kind = sg.kind_r & 7, r = sg.kind_r >> 3;
+    if (!s_go) break;  // thread 0 writes s_go again only after the barrier below
+    const BatchJob job = jobs[tr.x];
+    const int kind = tr.y & 7, r = tr.y >> 3;
     if (kind == TASK_DIAG) {
       const int base = r * BT;
       close_block64(job.mat + (int64_t)base * job.ld + base, job.ld, imin(BT, job.n - base), lds);
     } else {
-      batch_tile_task(job, kind, r, ti - sg.first, lds);
+      batch_tile_task(job, kind, r, tr.z, lds);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete in L2
+    if (t == 0) s_next = nxt;
     __syncthreads();                // ... every wave's; the LDS buffer is free again
-    if (t == 0) __hip_atomic_fetch_add(&done[sg.job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ti = next;
+    if (t == 0) __hip_atomic_fetch_add(&done[tr.x], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ti = s_next;  // thread 0 writes it again only after the next task
   }
 }
 
-// Task -> segment map of every queue: workgroup b writes segment b's
-// entries (segments of all queues, each queue's map at its own offset).
+// Task records of every queue: workgroup b writes segment b's (segments of
+// all queues, each queue's records from its own offset).
 __global__ void fw_queue_map_kernel(const BatchQSeg *__restrict__ all_segs,
-                                    const int32_t *__restrict__ seg_map_off,
-                                    const int32_t *__restrict__ seg_local, int nseg,
-                                    int32_t *segmap) {
+                                    const int32_t *__restrict__ seg_rec_off, int nseg,
+                                    int4 *rec) {
   for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
     const BatchQSeg sg = all_segs[b];
-    int32_t *m = segmap + seg_map_off[b] + sg.first;
-    for (int i = threadIdx.x; i < sg.count; i += blockDim.x) m[i] = seg_local[b];
+    int4 *m = rec + seg_rec_off[b] + sg.first;
+    for (int i = threadIdx.x; i < sg.count; i += blockDim.x)
+      m[i] = make_int4(sg.job, sg.kind_r, i, sg.need);
   }
 }
 
@@ -392,8 +390,8 @@ static int device_xcc_mask(fap_ctx *ctx, hipStream_t s, int *mask) {
 // Host side of a persistent launch (fw_queue_kernel): jobs split over the
 // device's XCDs by work (2 n^3, largest first, each to the least loaded),
 // one queue per XCD; one device block holds
-//   jobs | segments | qinfo | segment map offsets | local segment ids
-//   (uploaded) | ctrl | done | task -> segment map.
+//   jobs | segments | qinfo | each segment's first task record (uploaded)
+//   | ctrl | done | task records.
 int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const int32_t *n,
                       const int64_t *ld, hipStream_t s, QueueBlock *qb) {
   // tasks of one phase write distinct tiles, and a tile's rows must not
@@ -430,7 +428,7 @@ int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const
   // concatenated over the queues (segment job indices made global)
   std::vector<fapk::BatchJob> jobs;
   std::vector<fapk::BatchQSeg> segs;
-  std::vector<int32_t> qinfo(16 + 4 * nq, 0), seg_map_off, seg_local;
+  std::vector<int32_t> qinfo(16 + 4 * nq, 0), seg_rec_off;
   qb->job_total.clear();
   qb->ops = 0;
   for (int x = 0; x < 16; x++) qinfo[x] = -1;
@@ -464,8 +462,7 @@ int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const
     for (fapk::BatchQSeg g : qs) {
       g.job += job_base;
       qb->job_total[g.job] += g.count;
-      seg_map_off.push_back((int32_t)ntasks);
-      seg_local.push_back((int32_t)(segs.size() - qinfo[16 + 4 * q]));
+      seg_rec_off.push_back((int32_t)ntasks);
       segs.push_back(g);
     }
     ntasks += qt;
@@ -478,33 +475,30 @@ int build_queue_block(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, const
   const size_t jobs_b = sizeof(fapk::BatchJob) * jobs.size();
   qb->seg_off = up(jobs_b);
   qb->qi_off = up(qb->seg_off + sizeof(fapk::BatchQSeg) * segs.size());
-  qb->smo_off = up(qb->qi_off + sizeof(int32_t) * qinfo.size());
-  qb->sl_off = up(qb->smo_off + sizeof(int32_t) * segs.size());
-  qb->ctrl_off = up(qb->sl_off + sizeof(int32_t) * segs.size());
+  qb->sro_off = up(qb->qi_off + sizeof(int32_t) * qinfo.size());
+  qb->ctrl_off = up(qb->sro_off + sizeof(int32_t) * segs.size());
   qb->done_off = qb->ctrl_off + sizeof(int32_t) * qb->nctrl;
   qb->map_off = up(qb->done_off + sizeof(int32_t) * std::max(qb->njobs, 1));
-  qb->total = qb->map_off + sizeof(int32_t) * (size_t)std::max<int64_t>(ntasks, 1);
+  qb->total = qb->map_off + sizeof(int4) * (size_t)std::max<int64_t>(ntasks, 1);
   qb->host.assign(qb->ctrl_off, 0);
   char *h = qb->host.data();
   if (jobs_b) memcpy(h, jobs.data(), jobs_b);
   if (!segs.empty()) {
     memcpy(h + qb->seg_off, segs.data(), sizeof(fapk::BatchQSeg) * segs.size());
-    memcpy(h + qb->smo_off, seg_map_off.data(), sizeof(int32_t) * segs.size());
-    memcpy(h + qb->sl_off, seg_local.data(), sizeof(int32_t) * segs.size());
+    memcpy(h + qb->sro_off, seg_rec_off.data(), sizeof(int32_t) * segs.size());
   }
   memcpy(h + qb->qi_off, qinfo.data(), sizeof(int32_t) * qinfo.size());
   qb->grid = ctx->num_cus * 6;  // 6 resident workgroups per CU; dealt to the XCDs in turn
   return FAP_OK;
 }
 
-// Descriptors up and the task -> segment map filled (once per block).
+// Descriptors up and the task records filled (once per block).
 int upload_queue_block(const QueueBlock &qb, char *dev, hipStream_t s) {
   FAP_HIP(hipMemcpyAsync(dev, qb.host.data(), qb.ctrl_off, hipMemcpyHostToDevice, s));
   if (qb.nseg > 0) {
     hipLaunchKernelGGL(fapk::fw_queue_map_kernel, dim3(std::min(qb.nseg, 65535)), dim3(256), 0, s,
                        (const fapk::BatchQSeg *)(dev + qb.seg_off),
-                       (const int32_t *)(dev + qb.smo_off), (const int32_t *)(dev + qb.sl_off),
-                       qb.nseg, (int32_t *)(dev + qb.map_off));
+                       (const int32_t *)(dev + qb.sro_off), qb.nseg, (int4 *)(dev + qb.map_off));
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;
@@ -519,9 +513,8 @@ int enqueue_queue_run(const QueueBlock &qb, char *dev, hipStream_t s) {
                      dim3(256), 0, s, ctrl, qb.nctrl, done, qb.njobs);
   FAP_CHECK_LAUNCH();
   hipLaunchKernelGGL(fapk::fw_queue_kernel, dim3(qb.grid), dim3(fapk::MP_THREADS), 0, s,
-                     (const fapk::BatchJob *)dev, (const fapk::BatchQSeg *)(dev + qb.seg_off),
-                     (const int32_t *)(dev + qb.qi_off), (const int32_t *)(dev + qb.map_off), ctrl,
-                     done, qb.limit);
+                     (const fapk::BatchJob *)dev, (const int32_t *)(dev + qb.qi_off),
+                     (const int4 *)(dev + qb.map_off), ctrl, done, qb.limit);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }
@@ -763,7 +756,7 @@ int fw_plan_create(fap_ctx *ctx, int32_t count, int32_t *const *mats, const int3
 }
 
 // A plan of one persistent launch per run: the queue block built and
-// uploaded (task -> segment map filled) once; with use_graph the reset and
+// uploaded (task records filled) once; with use_graph the reset and
 // the launch are captured.
 int fw_plan_create_persistent(fap_ctx *ctx, int32_t count, int32_t *const *mats,
                               const int32_t *n, const int64_t *ld, int use_graph,

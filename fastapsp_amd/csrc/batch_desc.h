@@ -164,8 +164,8 @@ int32_t build_fw_queue(const FwBatchDesc &desc, std::vector<fapk::BatchQSeg> *ou
 // block) and the block's layout.
 struct QueueBlock {
   std::vector<char> host;
-  size_t seg_off = 0, qi_off = 0, smo_off = 0, sl_off = 0, ctrl_off = 0, done_off = 0,
-         map_off = 0, total = 0;
+  size_t seg_off = 0, qi_off = 0, sro_off = 0, ctrl_off = 0, done_off = 0, map_off = 0,
+         total = 0;
   int nctrl = 0, njobs = 0, nseg = 0, grid = 0;
   int64_t ntasks = 0, ops = 0, limit = 0, wait_us = 0;
   std::vector<int32_t> job_total;  // tasks per job (block order)
