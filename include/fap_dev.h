@@ -185,8 +185,8 @@ int fap_fw_batched_noalloc(fap_ctx *ctx, int32_t count, int32_t *const *d_mats, 
  * queue's XCD ran no workgroup).  Rows must be 128-byte aligned (each
  * matrix's address a multiple of 128 bytes, ld a multiple of 32), else
  * FAP_E_ARG: matrices on different XCDs then never share a cache line.
- * Slower than the plans' phase launches on the 512-graph batch (23 vs
- * 17 ms).  Synchronous. */
+ * Slower than the plans' phase launches on the 512-graph batch (22.6 vs
+ * 16.5 ms).  Synchronous. */
 int fap_fw_batched_persistent(fap_ctx *ctx, int32_t count, int32_t *const *d_mats,
                               const int32_t *n, const int64_t *ld, void *stream);
 
