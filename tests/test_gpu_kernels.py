@@ -39,7 +39,8 @@ def unpad(dev, rows, cols):
 
 MP_SHAPES = [(1, 1, 1), (5, 7, 3), (128, 128, 16), (129, 257, 17), (64, 300, 2), (300, 1000, 124),
              (257, 129, 250), (33, 4099, 61), (200, 300, 115), (70, 140, 15), (9, 600, 31),
-             (909, 1100, 115), (192, 200, 40)]   # 64-row tails: 13 rows, exactly 64 rows
+             (909, 1100, 115), (192, 200, 40),   # 64-row tails: 13 rows, exactly 64 rows
+             (16, 300, 40), (144, 1000, 7), (130, 257, 70), (1, 5000, 200)]  # up to 16 rows
 
 
 @pytest.mark.parametrize("M,N,K", MP_SHAPES)
@@ -67,7 +68,9 @@ def test_minplus_golden(gpu_ctx):
 
 @pytest.mark.parametrize("M,N,K,lda,ldb,ldc", [(70, 90, 33, 35, 93, 91), (128, 256, 32, 36, 260, 264),
                                                 (17, 130, 5, 5, 131, 133),
-                                                (150, 90, 33, 35, 93, 91)])
+                                                (150, 90, 33, 35, 93, 91),
+                                                (12, 300, 130, 131, 301, 303),
+                                                (141, 90, 70, 71, 93, 91)])
 def test_minplus_strided_accumulate(gpu_ctx, M, N, K, lda, ldb, ldc):
     A = rand_mat(M, K, 101)
     B = rand_mat(K, N, 102)
