@@ -191,7 +191,7 @@ __global__ void xcc_probe_kernel(int32_t *mask) {
 // kinds in one kernel take 80 VGPRs (64 spill): 6 waves per SIMD.
 //   qinfo: [0, 16) qmap (XCC id -> queue or -1); then per queue q, at
 //          16 + 4q: first segment, segment count, task count, first record
-//   taskrec: per task {job, kind_r, index in its phase, need}, filled by
+//   all_rec: per task {job, kind_r, index in its phase, need}, filled by
 //          fw_queue_map_kernel (one load per claimed task)
 //   ctrl:  [0] flag; claim counter of queue q at 32 (q + 1) (own cache line)
 __global__ __launch_bounds__(MP_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void fw_queue_kernel(
