@@ -108,11 +108,12 @@ def test_fw_plan_graph_replays_match_eager(gpu_ctx, sizes):
     plan = gpu_ctx.fw_plan(ptrs, sizes, lds, use_graph=True)
     info = plan.info()
     assert info["launches"] == _plan_launches(sizes)
+    plan.check()   # no run yet
     for rep in range(3):
         restore()
         gpu_ctx.sync()
         plan.run()
-        gpu_ctx.sync()
+        plan.check()   # a phase-launch plan: waits for the run, nothing to report
         for w, e in zip(work, eager):
             np.testing.assert_array_equal(w.get(), e, err_msg=f"replay {rep}")
     plan.close()
